@@ -1,0 +1,205 @@
+/*
+ * lssph.h -- C ABI of the MI355X (gfx950) LS-SPH hot path.
+ *
+ * One handle owns the device-resident particle state of one simulation, or of
+ * an ensemble of independent simulations that share the model options, plus
+ * its cell lists, ghost-wall markers, HIP stream and captured step graphs.
+ * Every entry point returns 0 on success and a negative LSSPH_E* code on
+ * failure; lssph_last_error() then holds a message.  Calls on one handle are
+ * asynchronous on the handle's stream except those that return host data
+ * (download, check_steady_state, neighbors, stats).  A handle is not
+ * reentrant: one host thread at a time.
+ *
+ * Array conventions: plain host pointers, element 0 = Fortran element 1.
+ * Particle arrays have n_total = sum(n_int) + sum(n_ext) entries ordered as
+ * the reference orders them for one box (source/setup/setup_closed_box_mod.f90:
+ * 185-193: all fluid particles first, then all ghost-wall particles); for an
+ * ensemble the fluid blocks of the instances come first, in instance order,
+ * followed by their wall blocks in instance order.  Virtual marker k pairs
+ * with wall particle sum(n_int) + k (source/setup/init_virtual_markers_mod.f90:
+ * 61-64).
+ */
+#ifndef LSSPH_H
+#define LSSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSSPH_OK 0
+#define LSSPH_E_ARG -1       /* invalid argument or unsupported option        */
+#define LSSPH_E_HIP -2       /* HIP runtime error (message has the HIP text)  */
+#define LSSPH_E_STATE -3     /* call out of order (e.g. step before upload)   */
+#define LSSPH_E_DOMAIN -4    /* a fluid particle left the cell grid           */
+#define LSSPH_E_NOMEM -5     /* device allocation failed                      */
+#define LSSPH_E_CAPACITY -6  /* caller buffer too small                       */
+
+/* precision codes */
+#define LSSPH_FP64 0   /* all FP64 (reference arithmetic)                     */
+#define LSSPH_MIXED 1  /* FP32 pair sums, FP64 factorisation + refinement     */
+
+/*
+ * Numeric parameters and model options of one instance: the C-interoperable
+ * part of Param_type (source/core/global_types.f90:22-73) plus the config.h
+ * model switches.  Derived fields (dx ... dt) are filled by the host exactly
+ * as init_param does (source/setup/init_param_mod.f90:92-199).  A Fortran
+ * caller mirrors it with a bind(C) derived type of the same field order.
+ */
+typedef struct lssph_params {
+  /* domain and resolution */
+  double len_x, len_y;      /* LEN_X, LEN_Y                                   */
+  double dx;                /* len_x / num_x                                  */
+  double sp_mass;           /* rho_ref * dx**2                                */
+  double h;                 /* coe_h * dx                                     */
+  double h_eff;             /* 2h (3h for the quintic spline)                 */
+  double w_ave;             /* W(dx, h)                                       */
+  double wl_thick;          /* num_wl * dx (0 when periodic)                  */
+  /* physics */
+  double rho_ref, vis_ref, sound_speed, thermal_dif;
+  double alpha_ref, gravity, tem_ave;
+  double tem_top, tem_bottom, tem_left, tem_right;
+  double u_top;
+  double zeta_rsst, xi_vim, coe_delta_sph, pst_c;
+  double dt;
+  double threshold;         /* steady-state threshold (diffusion test)        */
+  /* extensions (reference value in brackets) */
+  double diff_coef;         /* factor on the Laplacian in the diffusion RHS [1] */
+  double diff_source;       /* factor on the manufactured source term       [1] */
+  double cond_max;          /* mixed precision: condition estimate above which
+                               a particle is re-solved in full FP64 [1e5]     */
+  /* model options (config.h macros) */
+  int32_t target_problem;   /* TARGET_PROBLEM 1..4                            */
+  int32_t kernel_type;      /* KERNEL_TYPE 1..5                               */
+  int32_t sph_model;        /* SPH_MODEL: 3 (LS-SPH, 2nd-order Taylor)        */
+  int32_t wall_model;       /* WALL_MODEL 1..3                                */
+  int32_t rk;               /* RK 2 or 4                                      */
+  int32_t u_bc_top, u_bc_bottom, u_bc_left, u_bc_right;       /* U_BOUNDARY_*   */
+  int32_t tem_bc_top, tem_bc_bottom, tem_bc_left, tem_bc_right; /* TEM_BOUNDARY_* */
+  int32_t num_x, num_y, num_wl;
+  int32_t periodic;         /* 0: closed box with ghost walls (reference)
+                               1: doubly periodic box, no walls (extension)   */
+  int32_t precision;        /* LSSPH_FP64 or LSSPH_MIXED                      */
+} lssph_params_t;
+
+/* Counters the device keeps; read with lssph_stats(). */
+typedef struct lssph_stats {
+  int64_t fp64_fallbacks;   /* mixed path: particles re-solved in FP64        */
+  int64_t list_overflows;   /* particles whose neighbor list exceeded the
+                               on-chip list and took the direct path          */
+  int32_t max_neighbors;    /* largest neighbor count seen                    */
+  int32_t max_cell_count;   /* most particles in one cell                     */
+  int32_t escaped;          /* fluid particles found outside the cell grid    */
+  int32_t pad;
+} lssph_stats_t;
+
+typedef struct lssph_handle *lssph_handle_t;
+
+/* Replaces: allocation in setup_closed_box (setup_closed_box_mod.f90:142-183),
+   init_virtual_markers (init_virtual_markers_mod.f90:37-48) and init_cell
+   (init_cell_mod.f90:46-60,99-100).  params[n_inst], n_int[n_inst],
+   n_ext[n_inst]; options must agree across instances. */
+int32_t lssph_create(const lssph_params_t *params, int32_t n_inst,
+                     const int32_t *n_int, const int32_t *n_ext,
+                     lssph_handle_t *out);
+int32_t lssph_destroy(lssph_handle_t h);
+
+/* Message of the last failure on h (or of the last failed lssph_create when
+   h is NULL).  Valid until the next call on h. */
+const char *lssph_last_error(lssph_handle_t h);
+
+/* Host -> device copy of SP (global_types.f90:78-85).  tem may be NULL unless
+   target_problem == 4.  Also derives the virtual markers from the wall
+   positions (init_virtual_markers_mod.f90:61-137). */
+int32_t lssph_upload(lssph_handle_t h, const int32_t *ptype, const double *x,
+                     const double *y, const double *u, const double *v,
+                     const double *rho, const double *pre, const double *tem);
+
+/* Device -> host copy of SP in the upload order (for write_data,
+   write_data_mod.f90:43-52).  Synchronises. */
+int32_t lssph_download(lssph_handle_t h, int32_t *ptype, double *x, double *y,
+                       double *u, double *v, double *rho, double *pre,
+                       double *tem);
+
+/* Replaces init_cell + update_cell (init_cell_mod.f90:23-184,
+   update_cell_mod.f90:21-97): builds the cell grid, the wall and marker cell
+   indices, and bins every particle. */
+int32_t lssph_init_cell(lssph_handle_t h);
+
+/* Replaces update_cell (update_cell_mod.f90:21-97). */
+int32_t lssph_update_cell(lssph_handle_t h);
+
+/* Replaces ghost (ghost_mod.f90:43-78): marker interpolation + extrapolation
+   to the paired wall particles.  No-op for a periodic box. */
+int32_t lssph_ghost(lssph_handle_t h);
+
+/* Replaces calculate_RHS (calculate_RHS_mod.f90:29-52) / LSSPH_A
+   (LSSPH_A_mod.f90:26-271); results read back with lssph_download_rhs. */
+int32_t lssph_calculate_rhs(lssph_handle_t h);
+
+/* RHS_u, RHS_v, RHS_rho, RHS_tem of the fluid particles (sum(n_int) each, in
+   upload order).  rhs_tem may be NULL.  Synchronises. */
+int32_t lssph_download_rhs(lssph_handle_t h, double *rhs_u, double *rhs_v,
+                           double *rhs_rho, double *rhs_tem);
+
+/* Replace RK2 (RK2_mod.f90:30-128) and RK4 (RK4_mod.f90:30-179): one time
+   step with the per-stage update_cell and ghost. */
+int32_t lssph_rk2(lssph_handle_t h);
+int32_t lssph_rk4(lssph_handle_t h);
+
+/* Replaces PST (PST_mod.f90:28-116) incl. shift_interpolate
+   (shift_interpolate_mod.f90:23-300), its update_cell and ghost. */
+int32_t lssph_pst(lssph_handle_t h);
+
+/* nsteps iterations of the time loop body of main.f90:119-134 (RK2 or RK4
+   as params.rk says, then PST unless target_problem == 1), replayed from a
+   captured HIP graph. */
+int32_t lssph_step(lssph_handle_t h, int32_t nsteps);
+
+/* Replaces check_steady_state (check_steady_state_mod.f90:23-69): computes
+   max|f_previous - u| per instance; for instances not yet steady copies u to
+   f_previous.  is_steady[n_inst], df_max[n_inst] (either may be NULL).
+   Synchronises. */
+int32_t lssph_check_steady_state(lssph_handle_t h, int32_t *is_steady,
+                                 double *df_max);
+
+/* Virtual-marker data as write_param writes VM.dat (write_param_mod.f90:
+   147-157); sum(n_ext) entries each, tem may be NULL.  Synchronises. */
+int32_t lssph_download_vm(lssph_handle_t h, int32_t *ptype, double *x,
+                          double *y, double *u, double *v, double *pre,
+                          double *tem);
+
+/* cell%idx_SP (1-based cell index within the particle's own instance grid),
+   as write_param writes cell_idx.dat (write_param_mod.f90:133-136). */
+int32_t lssph_download_cell_index(lssph_handle_t h, int32_t *idx_sp);
+
+/* Validation: neighbor sets N(i) = { j != i in the 3x3 cell stencil with
+   r_ij^2 <= h_eff^2 } of every fluid particle (LSSPH_A_mod.f90:102-124), in
+   summation order, as 0-based upload-order indices.  counts[sum(n_int)];
+   list receives the concatenation (capacity entries available); *total gets
+   the number written.  Synchronises. */
+int32_t lssph_neighbors(lssph_handle_t h, int32_t *counts, int32_t *list,
+                        int64_t capacity, int64_t *total);
+
+/* Device counters (see lssph_stats_t).  Synchronises. */
+int32_t lssph_stats(lssph_handle_t h, lssph_stats_t *out);
+int32_t lssph_reset_stats(lssph_handle_t h);
+
+/* Waits for all work queued on the handle's stream. */
+int32_t lssph_synchronize(lssph_handle_t h);
+
+/* Tuning knobs (project autotuner): threads per block of the pair kernels,
+   on-chip neighbor-list length, and how many steps pass between physical
+   re-sorts of the particle records.  0 leaves a value unchanged. */
+int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len,
+                         int32_t reorder_interval);
+
+/* The HIP stream (hipStream_t) the handle launches on, for event timing. */
+void *lssph_stream(lssph_handle_t h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSSPH_H */
