@@ -1,0 +1,211 @@
+"""Throughput of the LS-SPH hot path on one MI355X.
+
+Workload (the headline configuration): doubly periodic Taylor-Green vortex,
+2048 x 2048 = 4,194,304 fluid particles, lattice jitter +-0.2 dx (SplitMix64
+seed 2029), Re = 1000, support radius 3.1 dx (Wendland C2, h = 1.55 dx),
+2nd-order LS (q = 5), RK4 with particle shifting every step, dt = 0.5 dx / c.
+A step is one RK4 + PST pass over all particles; inputs are resident in HBM
+before the timed region.  The mixed-precision path (FP32 pair sums, FP64
+factorisation + one refinement step, FP64 positions and RK accumulators) is
+the headline; the all-FP64 path runs the same steps for comparison, and the
+Taylor-Green L2 velocity errors of both against the analytic decay are
+reported.
+
+Prints one JSON line.  Usage: python bench.py [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "ls-sph-benchmarks_amd"))
+sys.path.insert(0, str(REPO))
+
+import numpy as np  # noqa: E402
+
+from lssph import workloads  # noqa: E402
+from lssph.engine import simulation_from_state  # noqa: E402
+
+METRIC = "particle-steps/s @4.2M particles RK4: mixed-prec vs FP64 GPU at equal L2 error"
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+VALU_FP64_TFLOPS = 78.6    # MI355X vector FP64 (= non-packed FP32 issue rate)
+# Compulsory FP64 state traffic of one fused LS type-A + RK4 stage launch,
+# per particle: stages 1..4 move 3, 5, 5, 3 fields-sets of F_e = 5 evolved
+# scalars x 8 B, i.e. 160 B averaged over the four stages.
+LSA_BYTES_PER_PARTICLE = 8.0 * 5 * (3 + 5 + 5 + 3) / 4.0
+# Pair arithmetic of one LS type-A pass per particle: 5 flops per candidate
+# (86.5) + ~101 per accepted neighbor (29.2) + ~303 for the solve and RHS.
+LSA_FLOPS_PER_PARTICLE = 5 * 86.5 + 101 * 29.2 + 303
+
+
+class HipEvent:
+    _hip = None
+
+    def __init__(self):
+        if HipEvent._hip is None:
+            HipEvent._hip = ctypes.CDLL("libamdhip64.so")
+        self.ev = ctypes.c_void_p()
+        if self._hip.hipEventCreate(ctypes.byref(self.ev)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+
+    def record(self, stream):
+        if self._hip.hipEventRecord(self.ev, ctypes.c_void_p(stream)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed_ms(self, end: "HipEvent") -> float:
+        self._hip.hipEventSynchronize(end.ev)
+        ms = ctypes.c_float()
+        if self._hip.hipEventElapsedTime(ctypes.byref(ms), self.ev, end.ev) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+
+def run_gpu(param, st, precision: int, steps: int, warmup: int) -> dict:
+    print(f"[bench] precision={'mixed' if precision else 'fp64'}: upload {st.num_int} particles", flush=True)
+    sim = simulation_from_state(param, st, precision=precision)
+    sim.eager_step(warmup)
+    sim.synchronize()
+    sim.kernel_times()
+    sim.reset_stats()
+    sim.profile(True)
+    e0, e1 = HipEvent(), HipEvent()
+    t0 = time.perf_counter()
+    e0.record(sim.stream)
+    sim.eager_step(steps)
+    e1.record(sim.stream)
+    ms = e0.elapsed_ms(e1)
+    wall = time.perf_counter() - t0
+    sim.profile(False)
+    kt = sim.kernel_times()
+    stats = sim.stats()
+    out = sim.download()
+    sim.close()
+    t = (warmup + steps) * param.dt
+    l2 = workloads.tg_l2_error(param, out, t, st.num_int)
+    print(f"[bench]   {steps} steps in {ms:.1f} ms (host wall {wall * 1e3:.1f} ms), L2={l2:.4e}", flush=True)
+    return {"ms": ms, "kt": kt, "stats": stats, "state": out, "l2": l2}
+
+
+def cpu_baseline(threads: int) -> dict | None:
+    """The reference Fortran (oracle/_ref/tg2048_bench, built from the
+    reference sources with MKL) timed on this host; falls back to the C port
+    of the oracle when that executable cannot run here."""
+    exe = REPO / "oracle" / "_ref" / "tg2048_bench" / "ref_harness"
+    if exe.exists():
+        with tempfile.TemporaryDirectory() as td:
+            try:
+                print(f"[bench] cpu baseline: reference Fortran, {threads} threads", flush=True)
+                r = subprocess.run([str(exe), td, "2", "0", str(threads)], capture_output=True, text=True,
+                                   timeout=240, cwd=td,
+                                   env=dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores"))
+                m = re.search(r"particle_steps_per_s\s+(\S+)", r.stdout)
+                if r.returncode == 0 and m:
+                    return {"value": float(m.group(1)), "unit": "particle-steps/s", "cores": threads,
+                            "kind": "reference",
+                            "sample": "reference Fortran+MKL, 2048x2048 Taylor-Green in the closed box with "
+                                      "free-slip ghost walls, jitter 0.2 dx, RK4+PST, 2 timed steps "
+                                      "(loop only, main.f90:110,162 timer)"}
+                print(f"[bench] reference run failed (rc={r.returncode}): {r.stderr[-300:]}", flush=True)
+            except (OSError, subprocess.TimeoutExpired) as e:
+                print(f"[bench] reference run failed: {e}", flush=True)
+    try:
+        from oracle.pyoracle import OracleSim
+        param, st = workloads.taylor_green_periodic(n=512, re=1000.0, rk=4, jitter=0.2, seed=2029)
+        OracleSim.set_threads(threads)
+        o = OracleSim(param.to_c(), st.num_int, st.num_ext, native=True)
+        o.init(st)
+        o.step(1)
+        t0 = time.perf_counter()
+        o.step(3)
+        dt = time.perf_counter() - t0
+        return {"value": st.num_int * 3 / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                "sample": "C port of the reference loops, 512x512 periodic Taylor-Green, RK4+PST, 3 steps"}
+    except Exception as e:  # noqa: BLE001
+        print(f"[bench] port baseline failed: {e}", flush=True)
+        return None
+
+
+def field_diff(a: dict, b: dict, n: int) -> dict:
+    out = {}
+    for k in ("x", "y", "u", "v", "rho", "pre"):
+        d = a[k][:n] - b[k][:n]
+        if k in ("x", "y"):
+            d = (d + 0.5) % 1.0 - 0.5  # periodic positions
+        out[k] = {"max": float(np.max(np.abs(d))), "l2": float(np.sqrt(np.mean(d * d)))}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=2048, help="particles per side")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the domain-decomposition design is described in DESIGN.md); run with --gpus 1.")
+        sys.exit(2)
+
+    n = args.n
+    param, st = workloads.taylor_green_periodic(n=n, re=1000.0, rk=4, jitter=0.2, seed=2029)
+    n_part = st.num_int
+    mixed = run_gpu(param, st, 1, args.steps, args.warmup)
+    fp64 = None if args.no_fp64 else run_gpu(param, st, 0, args.steps, args.warmup)
+
+    value = n_part * args.steps / (mixed["ms"] / 1e3)
+    kt = mixed["kt"]
+    lsa_avg_ms = kt.lsa_ms / max(kt.lsa_launches, 1)
+    bytes_per_launch = LSA_BYTES_PER_PARTICLE * n_part
+    achieved = bytes_per_launch / (lsa_avg_ms / 1e3) / 1e9
+    valu_tflops = LSA_FLOPS_PER_PARTICLE * n_part / (lsa_avg_ms / 1e3) / 1e12
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_lsa (fused LS type-A operator + RK4 stage update)",
+                "kernel_avg_ms": round(lsa_avg_ms, 4), "launches": int(kt.lsa_launches),
+                "bytes_per_launch": bytes_per_launch,
+                "valu": {"achieved": round(valu_tflops, 3), "peak": VALU_FP64_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(valu_tflops / VALU_FP64_TFLOPS, 4),
+                         "flops_per_launch": LSA_FLOPS_PER_PARTICLE * n_part}}
+    step_ms = mixed["ms"] / args.steps
+    breakdown = {k: round(getattr(kt, k + "_ms") / args.steps, 4) for k in ("lsa", "pst", "cells", "ghost", "other")}
+    result = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"periodic Taylor-Green {n}x{n}, jitter 0.2dx, Re 1000, h_eff 3.1dx, "
+                               f"LS q=5, RK4 + PST every step, mixed precision",
+                   "particles": n_part, "dt": param.dt, "parallelism": "single GPU"},
+        "ms_per_step_by_kernel": breakdown,
+        "tg_l2_error": {"mixed": mixed["l2"]},
+        "stats": {"fp64_fallbacks": int(mixed["stats"].fp64_fallbacks),
+                  "list_overflows": int(mixed["stats"].list_overflows)},
+        "roofline": roofline,
+    }
+    if fp64 is not None:
+        v64 = n_part * args.steps / (fp64["ms"] / 1e3)
+        result["fp64"] = {"value": round(v64, 1), "ms_per_step": round(fp64["ms"] / args.steps, 4),
+                          "speedup_mixed_over_fp64": round(value / v64, 4)}
+        result["tg_l2_error"]["fp64"] = fp64["l2"]
+        result["tg_l2_error"]["rel_diff"] = abs(mixed["l2"] - fp64["l2"]) / fp64["l2"]
+        result["mixed_vs_fp64"] = field_diff(mixed["state"], fp64["state"], n_part)
+    result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_threads)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

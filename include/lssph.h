@@ -198,6 +198,23 @@ int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len,
 /* The HIP stream (hipStream_t) the handle launches on, for event timing. */
 void *lssph_stream(lssph_handle_t h);
 
+/* Per-kernel timing.  While enabled, eager calls (lssph_rk2/rk4/pst/ghost/
+   update_cell, not the graphs of lssph_step) record a HIP event pair on the
+   handle's stream around every launch of each kernel group;
+   lssph_kernel_times() synchronizes, returns the accumulated milliseconds and
+   launch counts per group, and clears them.  No reference counterpart (the
+   reference times only its whole loop, main.f90:110,162). */
+typedef struct lssph_kernel_times {
+  double lsa_ms;     /* fused LS type-A operator + RK stage update          */
+  double pst_ms;     /* particle shifting                                   */
+  double cells_ms;   /* cell-list build (5 kernels per build)               */
+  double ghost_ms;   /* ghost-wall markers                                  */
+  double other_ms;   /* max-speed reduction                                 */
+  int64_t lsa_launches, pst_launches, cells_launches, ghost_launches, other_launches;
+} lssph_kernel_times_t;
+int32_t lssph_profile(lssph_handle_t h, int32_t enable);
+int32_t lssph_kernel_times(lssph_handle_t h, lssph_kernel_times_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,735 @@
+// api.hip -- the C ABI of include/lssph.h: handle lifecycle, host<->device
+// transfers, and the call sequences of the reference's RK2/RK4/PST/ghost
+// subroutines expressed as kernel launches on the handle's stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace lssph;
+
+struct lssph_handle {
+  Device d;
+  hipStream_t stream = nullptr;
+  std::vector<lssph_params_t> params;
+  std::vector<InstDev> inst_host;
+  std::vector<int32_t> n_int, n_ext;
+  std::vector<int32_t> inst_of;  // upload index -> instance
+  int cur = 0;
+  bool uploaded = false, cells_ready = false;
+  std::string err;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_end[2] = {0, 0};
+  int reorder_interval = 50;
+  int64_t steps_since_reorder = 0;
+  std::vector<void *> allocs;
+  // per-kernel event timing (eager calls only)
+  bool profiling = false, capturing = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct EvPair { hipEvent_t a, b; int group; };
+  std::vector<EvPair> ev_pending;
+  double kt_ms[5] = {0, 0, 0, 0, 0};
+  int64_t kt_n[5] = {0, 0, 0, 0, 0};
+};
+
+static thread_local std::string g_create_error;
+
+namespace {
+
+int fail(lssph_handle *h, int code, const std::string &msg) {
+  if (h)
+    h->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define HIP_TRY(h, expr)                                                               \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(h, LSSPH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(lssph_handle *h, T **p, size_t count) {
+  void *v = nullptr;
+  hipError_t e = hipMalloc(&v, sizeof(T) * std::max<size_t>(count, 1));
+  if (e != hipSuccess) return fail(h, LSSPH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  h->allocs.push_back(v);
+  *p = static_cast<T *>(v);
+  return 0;
+}
+
+double kernel_norm(int kt, double h) {
+  switch (kt) {
+    case 1: return 5.0 / (14.0 * kPi * (h * h));
+    case 2: return 7.0 / (478.0 * kPi * (h * h));
+    case 3: return 7.0 / (4.0 * kPi * (h * h));
+    case 4: return 9.0 / (4.0 * kPi * (h * h));
+    default: return 39.0 / (14.0 * kPi * (h * h));
+  }
+}
+
+int validate(const lssph_params_t *p, int n_inst, const int32_t *n_int, const int32_t *n_ext, std::string &msg) {
+  const lssph_params_t &a = p[0];
+  if (a.target_problem < 1 || a.target_problem > 4) return msg = "target_problem must be 1..4", 1;
+  if (a.kernel_type < 1 || a.kernel_type > 5) return msg = "kernel_type must be 1..5", 1;
+  if (a.sph_model != 3) return msg = "sph_model must be 3 (LS-SPH, 2nd-order Taylor)", 1;
+  if (a.wall_model < 1 || a.wall_model > 3) return msg = "wall_model must be 1..3", 1;
+  if (a.rk != 2 && a.rk != 4) return msg = "rk must be 2 or 4", 1;
+  if (a.precision != LSSPH_FP64 && a.precision != LSSPH_MIXED) return msg = "precision must be 0 or 1", 1;
+  for (int k = 0; k < n_inst; ++k) {
+    const lssph_params_t &b = p[k];
+    if (b.target_problem != a.target_problem || b.kernel_type != a.kernel_type ||
+        b.sph_model != a.sph_model || b.wall_model != a.wall_model || b.rk != a.rk ||
+        b.periodic != a.periodic || b.precision != a.precision || b.u_bc_top != a.u_bc_top ||
+        b.u_bc_bottom != a.u_bc_bottom || b.u_bc_left != a.u_bc_left || b.u_bc_right != a.u_bc_right ||
+        b.tem_bc_top != a.tem_bc_top || b.tem_bc_bottom != a.tem_bc_bottom ||
+        b.tem_bc_left != a.tem_bc_left || b.tem_bc_right != a.tem_bc_right)
+      return msg = "model options differ between instances", 1;
+    if (n_int[k] < 0 || n_ext[k] < 0) return msg = "negative particle count", 1;
+    if (!(b.h > 0.0) || !(b.h_eff > 0.0) || !(b.dx > 0.0) || !(b.sp_mass > 0.0) || !(b.len_x > 0.0) ||
+        !(b.len_y > 0.0))
+      return msg = "h, h_eff, dx, sp_mass, len_x, len_y must be positive", 1;
+    if (b.periodic && n_ext[k] != 0) return msg = "a periodic instance has no wall particles", 1;
+    if (b.periodic && (std::floor(b.len_x / b.h_eff) < 3 || std::floor(b.len_y / b.h_eff) < 3))
+      return msg = "a periodic box needs at least 3 cells of width >= h_eff per direction", 1;
+  }
+  return 0;
+}
+
+InstDev make_inst(const lssph_params_t &p) {
+  InstDev I{};
+  I.h = p.h;
+  I.inv_h = 1.0 / p.h;
+  I.inv_h2 = I.inv_h / p.h;
+  I.inv_h_sq = I.inv_h * I.inv_h;
+  I.h_eff = p.h_eff;
+  I.h_eff_sq = p.h_eff * p.h_eff;
+  I.inv_h_eff = 1.0 / p.h_eff;
+  I.threshold = p.threshold;
+  I.sp_mass = p.sp_mass;
+  I.w_ave = p.w_ave;
+  I.wl_thick = p.wl_thick;
+  I.len_x = p.len_x;
+  I.len_y = p.len_y;
+  I.w_norm = kernel_norm(p.kernel_type, p.h);
+  I.dw_fac = p.kernel_type == 1 ? -3.0 / p.h : (p.kernel_type == 2 ? -5.0 / p.h : 1.0 / p.h);
+  I.rho_ref = p.rho_ref;
+  I.vis_ref = p.vis_ref;
+  I.c_sq = p.sound_speed * p.sound_speed;
+  I.thermal_dif = p.thermal_dif;
+  I.alpha_ref = p.alpha_ref;
+  I.gravity = p.gravity;
+  I.tem_ave = p.tem_ave;
+  I.zeta_sq = p.zeta_rsst * p.zeta_rsst;
+  I.xi_sq = p.xi_vim * p.xi_vim;
+  I.delta = p.coe_delta_sph * p.sound_speed * p.h * p.zeta_rsst;
+  I.pst_c = p.pst_c;
+  I.dt = p.dt;
+  I.u_top = p.u_top;
+  I.tem_top = p.tem_top;
+  I.tem_bottom = p.tem_bottom;
+  I.tem_left = p.tem_left;
+  I.tem_right = p.tem_right;
+  I.diff_coef = p.diff_coef;
+  I.diff_source = p.diff_source;
+  I.cond_max = p.cond_max > 0.0 ? p.cond_max : 1.0e4;
+  I.x_min = 0.0 + p.wl_thick;
+  I.x_max = p.len_x + p.wl_thick;
+  I.y_min = 0.0 + p.wl_thick;
+  I.y_max = p.len_y + p.wl_thick;
+  I.periodic = p.periodic;
+  if (p.periodic) {
+    I.nx = (int)std::floor(p.len_x / p.h_eff);
+    I.ny = (int)std::floor(p.len_y / p.h_eff);
+    I.inv_cw_x = (double)I.nx / p.len_x;
+    I.inv_cw_y = (double)I.ny / p.len_y;
+  } else {
+    // init_cell_mod.f90:36-41
+    I.nx = (int)std::ceil((p.len_x + 2.0 * p.wl_thick) / p.h_eff);
+    I.ny = (int)std::ceil((p.len_y + 2.0 * p.wl_thick) / p.h_eff);
+  }
+  return I;
+}
+
+int check_counters(lssph_handle *h) {
+  int32_t c[kNumCounters];
+  HIP_TRY(h, hipMemcpyAsync(c, h->d.counters, sizeof c, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (c[kEscaped] > 0) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%d particle(s) or marker(s) left the cell grid", c[kEscaped]);
+    return fail(h, LSSPH_E_DOMAIN, buf);
+  }
+  return 0;
+}
+
+bool is_diffusion(const lssph_handle *h) { return h->d.opt.target_problem == 1; }
+
+enum { kGrpLsa = 0, kGrpPst, kGrpCells, kGrpGhost, kGrpOther };
+
+hipEvent_t take_event(lssph_handle *h) {
+  if (!h->ev_pool.empty()) {
+    hipEvent_t e = h->ev_pool.back();
+    h->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets the launches made by f() with an event pair when profiling.
+template <typename F>
+void timed(lssph_handle *h, int group, F &&f) {
+  if (!h->profiling || h->capturing) {
+    f();
+    return;
+  }
+  hipEvent_t a = take_event(h), b = take_event(h);
+  if (!a || !b) {
+    f();
+    return;
+  }
+  (void)hipEventRecord(a, h->stream);
+  f();
+  (void)hipEventRecord(b, h->stream);
+  h->ev_pending.push_back({a, b, group});
+}
+
+// One RK stage pass: fused operator + update, then update_cell and ghost on
+// the new state (RK2_mod.f90:67-126, RK4_mod.f90:205-306).
+void enqueue_stage(lssph_handle *h, StageMode m) {
+  Device &d = h->d;
+  timed(h, kGrpLsa, [&] { launch_lsa(h->stream, d, h->cur, m); });
+  h->cur ^= 1;
+  if (!is_diffusion(h)) timed(h, kGrpCells, [&] { launch_bin(h->stream, d, d.rec[h->cur], 0); });
+  if (d.n_ext) timed(h, kGrpGhost, [&] { launch_ghost(h->stream, d, h->cur); });
+}
+
+void enqueue_rk(lssph_handle *h) {
+  if (h->d.opt.rk == 2) {
+    enqueue_stage(h, kRk2S1);
+    enqueue_stage(h, kRk2S2);
+  } else {
+    enqueue_stage(h, kRk4S1);
+    enqueue_stage(h, kRk4S2);
+    enqueue_stage(h, kRk4S3);
+    enqueue_stage(h, kRk4S4);
+  }
+}
+
+// PST_mod.f90:37-114
+void enqueue_pst(lssph_handle *h) {
+  Device &d = h->d;
+  timed(h, kGrpOther, [&] { launch_umax(h->stream, d, h->cur); });
+  timed(h, kGrpPst, [&] { launch_pst(h->stream, d, h->cur); });
+  h->cur ^= 1;
+  timed(h, kGrpCells, [&] { launch_bin(h->stream, d, d.rec[h->cur], 0); });
+  if (d.n_ext) timed(h, kGrpGhost, [&] { launch_ghost(h->stream, d, h->cur); });
+}
+
+void enqueue_step(lssph_handle *h) {
+  enqueue_rk(h);
+  if (!is_diffusion(h)) enqueue_pst(h);
+}
+
+// Physical re-sort: permutes every per-particle array into the current
+// (cell, upload index) order so that neighbor records stay contiguous.
+int reorder(lssph_handle *h) {
+  Device &d = h->d;
+  hipStream_t s = h->stream;
+  const int n = d.n_total;
+  const int other = h->cur ^ 1;
+  launch_gather_rec(s, n, d.items, d.rec[h->cur], d.rec[other]);
+  HIP_TRY(h, hipMemcpyAsync(d.rec[h->cur], d.rec[other], sizeof(Rec) * n, hipMemcpyDeviceToDevice, s));
+  launch_gather_i32(s, n, d.items, d.info, d.stage_i32);
+  HIP_TRY(h, hipMemcpyAsync(d.info, d.stage_i32, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  launch_gather_i32(s, n, d.items, d.key, d.stage_i32);
+  HIP_TRY(h, hipMemcpyAsync(d.key, d.stage_i32, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  launch_gather_i32(s, n, d.items, d.orig_of, d.stage_i32);
+  HIP_TRY(h, hipMemcpyAsync(d.orig_of, d.stage_i32, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  launch_gather_f64(s, n, d.items, d.f_prev, d.stage_f64);
+  HIP_TRY(h, hipMemcpyAsync(d.f_prev, d.stage_f64, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  launch_inverse_perm(s, n, d.orig_of, d.phys_of);
+  launch_vm_init(s, d, h->cur);
+  launch_bin(s, d, d.rec[h->cur], 0);
+  h->steps_since_reorder = 0;
+  return 0;
+}
+
+void drop_graphs(lssph_handle *h) {
+  for (int i = 0; i < 2; ++i)
+    if (h->graph[i]) {
+      (void)hipGraphExecDestroy(h->graph[i]);
+      h->graph[i] = nullptr;
+    }
+}
+
+int ready(lssph_handle *h) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!h->uploaded) return fail(h, LSSPH_E_STATE, "call lssph_upload first");
+  if (!h->cells_ready) return fail(h, LSSPH_E_STATE, "call lssph_init_cell first");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t *n_int, const int32_t *n_ext,
+                     lssph_handle_t *out) {
+  if (!params || !n_int || !n_ext || !out || n_inst < 1)
+    return fail(nullptr, LSSPH_E_ARG, "lssph_create: null argument or n_inst < 1");
+  *out = nullptr;
+  std::string msg;
+  if (validate(params, n_inst, n_int, n_ext, msg)) return fail(nullptr, LSSPH_E_ARG, "lssph_create: " + msg);
+  auto *h = new lssph_handle();
+  h->params.assign(params, params + n_inst);
+  h->n_int.assign(n_int, n_int + n_inst);
+  h->n_ext.assign(n_ext, n_ext + n_inst);
+  Device &d = h->d;
+  d.n_inst = n_inst;
+  int64_t ni = 0, ne = 0, nc = 0;
+  for (int k = 0; k < n_inst; ++k) {
+    InstDev I = make_inst(params[k]);
+    I.cell_off = (int32_t)nc;
+    nc += (int64_t)I.nx * I.ny;
+    h->inst_host.push_back(I);
+    ni += n_int[k];
+    ne += n_ext[k];
+  }
+  if (ni + ne > (1ll << 30) || nc > (1ll << 30) || n_inst >= (1 << 23)) {
+    delete h;
+    return fail(nullptr, LSSPH_E_ARG, "lssph_create: problem too large for 32-bit indices");
+  }
+  d.n_int = (int)ni;
+  d.n_ext = (int)ne;
+  d.n_total = (int)(ni + ne);
+  d.n_cells = (int)nc;
+  const lssph_params_t &a = params[0];
+  d.opt = Options{a.target_problem, a.kernel_type, a.wall_model, a.rk, a.u_bc_top, a.u_bc_bottom,
+                  a.u_bc_left, a.u_bc_right, a.tem_bc_top, a.tem_bc_bottom, a.tem_bc_left, a.tem_bc_right,
+                  a.target_problem == 4 ? 1 : 0, a.precision};
+  // upload index -> instance: fluid blocks, then wall blocks, in instance order
+  h->inst_of.resize(d.n_total);
+  {
+    int64_t f = 0, w = ni;
+    for (int k = 0; k < n_inst; ++k) {
+      for (int i = 0; i < n_int[k]; ++i) h->inst_of[f++] = k;
+      for (int i = 0; i < n_ext[k]; ++i) h->inst_of[w++] = k;
+    }
+  }
+  const size_t n = d.n_total;
+  int rc = 0;
+  rc = rc ? rc : dalloc(h, &d.inst, n_inst);
+  rc = rc ? rc : dalloc(h, &d.rec[0], n);
+  rc = rc ? rc : dalloc(h, &d.rec[1], n);
+  rc = rc ? rc : dalloc(h, &d.info, n);
+  rc = rc ? rc : dalloc(h, &d.key, n);
+  rc = rc ? rc : dalloc(h, &d.orig_of, n);
+  rc = rc ? rc : dalloc(h, &d.phys_of, n);
+  rc = rc ? rc : dalloc(h, &d.cell_count, d.n_cells);
+  rc = rc ? rc : dalloc(h, &d.cell_start, d.n_cells + 1);
+  rc = rc ? rc : dalloc(h, &d.items, n);
+  rc = rc ? rc : dalloc(h, &d.scan_tmp, scan_tile_count(d.n_cells) + 1);
+  rc = rc ? rc : dalloc(h, &d.perm_tmp, n);
+  for (int f = 0; f < 6 && !rc; ++f) rc = dalloc(h, &d.old_[f], n);
+  for (int f = 0; f < 6 && !rc; ++f) rc = dalloc(h, &d.new_[f], n);
+  for (int f = 0; f < 4 && !rc; ++f) rc = dalloc(h, &d.rhs[f], n);
+  rc = rc ? rc : dalloc(h, &d.f_prev, n);
+  const size_t m = d.n_ext;
+  rc = rc ? rc : dalloc(h, &d.vm_x, m);
+  rc = rc ? rc : dalloc(h, &d.vm_y, m);
+  rc = rc ? rc : dalloc(h, &d.vm_nx, m);
+  rc = rc ? rc : dalloc(h, &d.vm_ny, m);
+  rc = rc ? rc : dalloc(h, &d.vm_u, m);
+  rc = rc ? rc : dalloc(h, &d.vm_v, m);
+  rc = rc ? rc : dalloc(h, &d.vm_pre, m);
+  rc = rc ? rc : dalloc(h, &d.vm_tem, m);
+  rc = rc ? rc : dalloc(h, &d.vm_ptype, m);
+  rc = rc ? rc : dalloc(h, &d.vm_inst, m);
+  rc = rc ? rc : dalloc(h, &d.vm_cell, m);
+  rc = rc ? rc : dalloc(h, &d.vm_wall, m);
+  rc = rc ? rc : dalloc(h, &d.umax, n_inst);
+  rc = rc ? rc : dalloc(h, &d.dfmax, n_inst);
+  rc = rc ? rc : dalloc(h, &d.steady, n_inst);
+  rc = rc ? rc : dalloc(h, &d.counters, kNumCounters);
+  rc = rc ? rc : dalloc(h, &d.stats64, kNumStats64);
+  rc = rc ? rc : dalloc(h, &d.stage_f64, 8 * n);
+  rc = rc ? rc : dalloc(h, &d.stage_i32, n);
+  if (!rc) {
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) rc = fail(h, LSSPH_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  if (!rc) {
+    hipError_t e = hipMemcpy(d.inst, h->inst_host.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(h, LSSPH_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
+  if (rc) {
+    g_create_error = h->err;
+    lssph_destroy(h);
+    return rc;
+  }
+  launch_fill_i32(h->stream, d.cell_count, 0, d.n_cells);
+  launch_fill_i32(h->stream, d.counters, 0, kNumCounters);
+  launch_fill_u64(h->stream, d.stats64, 0ull, kNumStats64);
+  *out = h;
+  return 0;
+}
+
+int32_t lssph_destroy(lssph_handle_t h) {
+  if (!h) return 0;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graphs(h);
+  for (auto &e : h->ev_pending) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+  for (void *p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+const char *lssph_last_error(lssph_handle_t h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int32_t lssph_upload(lssph_handle_t h, const int32_t *ptype, const double *x, const double *y, const double *u,
+                     const double *v, const double *rho, const double *pre, const double *tem) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!ptype || !x || !y || !u || !v || !rho || !pre) return fail(h, LSSPH_E_ARG, "lssph_upload: null array");
+  Device &d = h->d;
+  const int has_tem = d.opt.has_tem;
+  if (has_tem && !tem) return fail(h, LSSPH_E_ARG, "lssph_upload: tem is required for target_problem 4");
+  const size_t n = d.n_total;
+  for (size_t i = 0; i < n; ++i) {
+    const bool fluid_slot = (int64_t)i < d.n_int;
+    if ((ptype[i] == 0) != fluid_slot || ptype[i] < 0 || ptype[i] > 8)
+      return fail(h, LSSPH_E_ARG, "lssph_upload: ptype must be 0 for the fluid block and 1..8 for walls");
+    if (!(rho[i] > 0.0)) return fail(h, LSSPH_E_ARG, "lssph_upload: rho must be positive");
+  }
+  hipStream_t s = h->stream;
+  drop_graphs(h);
+  const double *src[7] = {x, y, u, v, rho, pre, tem};
+  for (int f = 0; f < 7; ++f)
+    if (src[f]) HIP_TRY(h, hipMemcpyAsync(d.stage_f64 + f * n, src[f], sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(h, hipMemcpyAsync(d.stage_i32, ptype, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(h, hipMemcpyAsync(d.perm_tmp, h->inst_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  launch_upload(s, d, d.stage_f64, d.stage_i32, d.perm_tmp, has_tem);
+  // identity permutation, then the initial (cell, index) sort
+  std::vector<int32_t> iota(n);
+  for (size_t i = 0; i < n; ++i) iota[i] = (int32_t)i;
+  HIP_TRY(h, hipMemcpyAsync(d.orig_of, iota.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(h, hipStreamSynchronize(s));  // iota is a host temporary
+  launch_fill_i32(s, d.counters, 0, kNumCounters);
+  launch_bin(s, d, d.rec[0], 1);
+  h->cur = 0;
+  HIP_TRY(h, hipMemcpyAsync(d.rec[1], d.rec[0], sizeof(Rec) * n, hipMemcpyDeviceToDevice, s));
+  // f_prev = 1 (setup_closed_box_mod.f90:180-183); permuted along with the rest
+  {
+    std::vector<double> ones(n, 1.0);
+    HIP_TRY(h, hipMemcpyAsync(d.f_prev, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(h, hipStreamSynchronize(s));
+  }
+  launch_inverse_perm(s, d.n_total, d.orig_of, d.phys_of);
+  int rc = reorder(h);
+  if (rc) return rc;
+  h->uploaded = true;
+  h->cells_ready = false;
+  rc = check_counters(h);
+  if (rc) return rc;
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_download(lssph_handle_t h, int32_t *ptype, double *x, double *y, double *u, double *v, double *rho,
+                       double *pre, double *tem) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!h->uploaded) return fail(h, LSSPH_E_STATE, "call lssph_upload first");
+  Device &d = h->d;
+  hipStream_t s = h->stream;
+  const size_t n = d.n_total;
+  launch_download(s, d, h->cur);
+  double *dst[7] = {x, y, u, v, rho, pre, d.opt.has_tem ? tem : nullptr};
+  for (int f = 0; f < 7; ++f)
+    if (dst[f]) HIP_TRY(h, hipMemcpyAsync(dst[f], d.stage_f64 + f * n, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (ptype) HIP_TRY(h, hipMemcpyAsync(ptype, d.stage_i32, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(h, hipStreamSynchronize(s));
+  HIP_TRY(h, hipGetLastError());
+  return check_counters(h);
+}
+
+int32_t lssph_download_cell_index(lssph_handle_t h, int32_t *idx_sp) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!h->uploaded) return fail(h, LSSPH_E_STATE, "call lssph_upload first");
+  Device &d = h->d;
+  launch_download(h->stream, d, h->cur);
+  HIP_TRY(h, hipMemcpyAsync(idx_sp, d.perm_tmp, sizeof(int32_t) * d.n_total, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int32_t lssph_init_cell(lssph_handle_t h) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!h->uploaded) return fail(h, LSSPH_E_STATE, "call lssph_upload first");
+  launch_bin(h->stream, h->d, h->d.rec[h->cur], 0);
+  h->cells_ready = true;
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_update_cell(lssph_handle_t h) {
+  if (int rc = ready(h)) return rc;
+  launch_bin(h->stream, h->d, h->d.rec[h->cur], 0);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_ghost(lssph_handle_t h) {
+  if (int rc = ready(h)) return rc;
+  launch_ghost(h->stream, h->d, h->cur);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_calculate_rhs(lssph_handle_t h) {
+  if (int rc = ready(h)) return rc;
+  launch_lsa(h->stream, h->d, h->cur, kRhsOnly);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_download_rhs(lssph_handle_t h, double *rhs_u, double *rhs_v, double *rhs_rho, double *rhs_tem) {
+  if (int rc = ready(h)) return rc;
+  Device &d = h->d;
+  launch_download_rhs(h->stream, d);
+  double *dst[4] = {rhs_u, rhs_v, rhs_rho, d.opt.has_tem ? rhs_tem : nullptr};
+  for (int f = 0; f < 4; ++f)
+    if (dst[f])
+      HIP_TRY(h, hipMemcpyAsync(dst[f], d.stage_f64 + (size_t)f * d.n_int, sizeof(double) * d.n_int,
+                                hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int32_t lssph_rk2(lssph_handle_t h) {
+  if (int rc = ready(h)) return rc;
+  if (h->d.opt.rk != 2) return fail(h, LSSPH_E_ARG, "handle was created with rk = 4");
+  enqueue_rk(h);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_rk4(lssph_handle_t h) {
+  if (int rc = ready(h)) return rc;
+  if (h->d.opt.rk != 4) return fail(h, LSSPH_E_ARG, "handle was created with rk = 2");
+  enqueue_rk(h);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_pst(lssph_handle_t h) {
+  if (int rc = ready(h)) return rc;
+  enqueue_pst(h);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_step(lssph_handle_t h, int32_t nsteps) {
+  if (int rc = ready(h)) return rc;
+  if (nsteps < 0) return fail(h, LSSPH_E_ARG, "nsteps must be >= 0");
+  for (int32_t i = 0; i < nsteps; ++i) {
+    if (h->reorder_interval > 0 && !is_diffusion(h) && h->steps_since_reorder >= h->reorder_interval) {
+      if (int rc = reorder(h)) return rc;
+    }
+    const int c0 = h->cur;
+    if (!h->graph[c0]) {
+      hipGraph_t g = nullptr;
+      HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      h->capturing = true;
+      enqueue_step(h);
+      h->capturing = false;
+      const int c1 = h->cur;
+      h->cur = c0;
+      HIP_TRY(h, hipStreamEndCapture(h->stream, &g));
+      hipError_t e = hipGraphInstantiate(&h->graph[c0], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        h->graph[c0] = nullptr;
+        return fail(h, LSSPH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+      h->graph_end[c0] = c1;
+    }
+    HIP_TRY(h, hipGraphLaunch(h->graph[c0], h->stream));
+    h->cur = h->graph_end[c0];
+    ++h->steps_since_reorder;
+  }
+  return 0;
+}
+
+int32_t lssph_check_steady_state(lssph_handle_t h, int32_t *is_steady, double *df_max) {
+  if (int rc = ready(h)) return rc;
+  Device &d = h->d;
+  launch_steady(h->stream, d, h->cur);
+  std::vector<unsigned long long> bits(d.n_inst);
+  std::vector<int32_t> st(d.n_inst);
+  HIP_TRY(h, hipMemcpyAsync(bits.data(), d.dfmax, sizeof(unsigned long long) * d.n_inst, hipMemcpyDeviceToHost,
+                            h->stream));
+  HIP_TRY(h, hipMemcpyAsync(st.data(), d.steady, sizeof(int32_t) * d.n_inst, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (int k = 0; k < d.n_inst; ++k) {
+    double v;
+    std::memcpy(&v, &bits[k], sizeof v);
+    if (df_max) df_max[k] = v;
+    if (is_steady) is_steady[k] = st[k];
+  }
+  return 0;
+}
+
+int32_t lssph_download_vm(lssph_handle_t h, int32_t *ptype, double *x, double *y, double *u, double *v,
+                          double *pre, double *tem) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!h->uploaded) return fail(h, LSSPH_E_STATE, "call lssph_upload first");
+  Device &d = h->d;
+  const size_t m = d.n_ext;
+  hipStream_t s = h->stream;
+  if (m == 0) return 0;
+  if (ptype) HIP_TRY(h, hipMemcpyAsync(ptype, d.vm_ptype, sizeof(int32_t) * m, hipMemcpyDeviceToHost, s));
+  const double *src[6] = {d.vm_x, d.vm_y, d.vm_u, d.vm_v, d.vm_pre, d.vm_tem};
+  double *dst[6] = {x, y, u, v, pre, d.opt.has_tem ? tem : nullptr};
+  for (int f = 0; f < 6; ++f)
+    if (dst[f]) HIP_TRY(h, hipMemcpyAsync(dst[f], src[f], sizeof(double) * m, hipMemcpyDeviceToHost, s));
+  HIP_TRY(h, hipStreamSynchronize(s));
+  return 0;
+}
+
+int32_t lssph_neighbors(lssph_handle_t h, int32_t *counts, int32_t *list, int64_t capacity, int64_t *total) {
+  if (int rc = ready(h)) return rc;
+  if (!counts) return fail(h, LSSPH_E_ARG, "lssph_neighbors: null counts");
+  Device &d = h->d;
+  hipStream_t s = h->stream;
+  int32_t *dcounts = nullptr;
+  int64_t *doff = nullptr;
+  int32_t *dlist = nullptr;
+  HIP_TRY(h, hipMalloc(&dcounts, sizeof(int32_t) * std::max(d.n_int, 1)));
+  launch_neighbor_count(s, d, h->cur, dcounts);
+  HIP_TRY(h, hipMemcpyAsync(counts, dcounts, sizeof(int32_t) * d.n_int, hipMemcpyDeviceToHost, s));
+  HIP_TRY(h, hipStreamSynchronize(s));
+  std::vector<int64_t> off(d.n_int + 1, 0);
+  for (int i = 0; i < d.n_int; ++i) off[i + 1] = off[i] + counts[i];
+  const int64_t tot = off[d.n_int];
+  if (total) *total = tot;
+  int rc = 0;
+  if (tot > capacity || !list) {
+    rc = fail(h, LSSPH_E_CAPACITY, "lssph_neighbors: list capacity too small");
+  } else if (tot > 0) {
+    if (hipMalloc(&doff, sizeof(int64_t) * (d.n_int + 1)) != hipSuccess ||
+        hipMalloc(&dlist, sizeof(int32_t) * tot) != hipSuccess) {
+      rc = fail(h, LSSPH_E_NOMEM, "lssph_neighbors: hipMalloc failed");
+    } else {
+      (void)hipMemcpyAsync(doff, off.data(), sizeof(int64_t) * (d.n_int + 1), hipMemcpyHostToDevice, s);
+      launch_neighbor_fill(s, d, h->cur, doff, dlist);
+      (void)hipMemcpyAsync(list, dlist, sizeof(int32_t) * tot, hipMemcpyDeviceToHost, s);
+      if (hipStreamSynchronize(s) != hipSuccess) rc = fail(h, LSSPH_E_HIP, "lssph_neighbors: kernel failed");
+    }
+  }
+  (void)hipFree(dcounts);
+  if (doff) (void)hipFree(doff);
+  if (dlist) (void)hipFree(dlist);
+  return rc;
+}
+
+int32_t lssph_stats(lssph_handle_t h, lssph_stats_t *out) {
+  if (!h || !out) return fail(h, LSSPH_E_ARG, "lssph_stats: null argument");
+  int32_t c[kNumCounters];
+  unsigned long long s64[kNumStats64];
+  HIP_TRY(h, hipMemcpyAsync(c, h->d.counters, sizeof c, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(s64, h->d.stats64, sizeof s64, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  out->fp64_fallbacks = (int64_t)s64[kFallbacks];
+  out->list_overflows = (int64_t)s64[kOverflows];
+  out->max_neighbors = c[kMaxNbr];
+  out->max_cell_count = c[kMaxCell];
+  out->escaped = c[kEscaped];
+  out->pad = 0;
+  return 0;
+}
+
+int32_t lssph_reset_stats(lssph_handle_t h) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  launch_fill_u64(h->stream, h->d.stats64, 0ull, kNumStats64);
+  launch_fill_i32(h->stream, h->d.counters + kMaxCell, 0, kNumCounters - kMaxCell);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_synchronize(lssph_handle_t h) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipGetLastError());
+  return check_counters(h);
+}
+
+int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len, int32_t reorder_interval) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (block) {
+    if (block < 64 || block > 256 || block % 64)
+      return fail(h, LSSPH_E_ARG, "block must be 64, 128, 192 or 256");
+    h->d.block = block;
+  }
+  if (list_len) {
+    if (list_len < 8 || list_len > 128) return fail(h, LSSPH_E_ARG, "list_len must be in [8, 128]");
+    if ((size_t)list_len * (block ? block : h->d.block) * 4 > 64 * 1024)
+      return fail(h, LSSPH_E_ARG, "list_len * block exceeds 64 KiB of LDS");
+    h->d.list_len = list_len;
+  }
+  if (reorder_interval) h->reorder_interval = reorder_interval < 0 ? 0 : reorder_interval;
+  drop_graphs(h);
+  return 0;
+}
+
+void *lssph_stream(lssph_handle_t h) { return h ? (void *)h->stream : nullptr; }
+
+int32_t lssph_profile(lssph_handle_t h, int32_t enable) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  h->profiling = enable != 0;
+  return 0;
+}
+
+int32_t lssph_kernel_times(lssph_handle_t h, lssph_kernel_times_t *out) {
+  if (!h || !out) return fail(h, LSSPH_E_ARG, "lssph_kernel_times: null argument");
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (auto &e : h->ev_pending) {
+    float ms = 0.0f;
+    HIP_TRY(h, hipEventElapsedTime(&ms, e.a, e.b));
+    h->kt_ms[e.group] += ms;
+    h->kt_n[e.group] += 1;
+    h->ev_pool.push_back(e.a);
+    h->ev_pool.push_back(e.b);
+  }
+  h->ev_pending.clear();
+  out->lsa_ms = h->kt_ms[kGrpLsa];
+  out->pst_ms = h->kt_ms[kGrpPst];
+  out->cells_ms = h->kt_ms[kGrpCells];
+  out->ghost_ms = h->kt_ms[kGrpGhost];
+  out->other_ms = h->kt_ms[kGrpOther];
+  out->lsa_launches = h->kt_n[kGrpLsa];
+  out->pst_launches = h->kt_n[kGrpPst];
+  out->cells_launches = h->kt_n[kGrpCells];
+  out->ghost_launches = h->kt_n[kGrpGhost];
+  out->other_launches = h->kt_n[kGrpOther];
+  for (int g = 0; g < 5; ++g) {
+    h->kt_ms[g] = 0.0;
+    h->kt_n[g] = 0;
+  }
+  return 0;
+}
+
+}  // extern "C"

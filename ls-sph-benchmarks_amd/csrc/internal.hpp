@@ -1,0 +1,81 @@
+// internal.hpp -- device-side state of a handle and the kernel launchers
+// shared by the translation units of liblssph_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/lssph.h"
+#include "device_common.hpp"
+
+namespace lssph {
+
+// Indices into Device::counters (int32) and Device::stats64 (uint64).
+enum { kEscaped = 0, kMaxCell = 1, kMaxNbr = 2, kNumCounters = 4 };
+enum { kFallbacks = 0, kOverflows = 1, kNumStats64 = 2 };
+
+// Raw device pointers and sizes; passed to the launchers.
+struct Device {
+  int n_total = 0, n_int = 0, n_ext = 0, n_inst = 0, n_cells = 0;
+  Options opt{};
+  InstDev *inst = nullptr;          // [n_inst]
+  Rec *rec[2] = {nullptr, nullptr};  // double-buffered records, physical order
+  int32_t *info = nullptr;          // [n_total] ptype | inst << 8
+  int32_t *key = nullptr;           // [n_total] global cell id
+  int32_t *orig_of = nullptr;       // [n_total] physical -> upload index
+  int32_t *phys_of = nullptr;       // [n_total] upload index -> physical
+  int32_t *cell_count = nullptr;    // [n_cells]
+  int32_t *cell_start = nullptr;    // [n_cells + 1]
+  int32_t *items = nullptr;         // [n_total] physical ids sorted by cell
+  int32_t *scan_tmp = nullptr;      // scan tile sums
+  int32_t *perm_tmp = nullptr;      // [n_total]
+  // RK work arrays (physical order, fluid entries used)
+  double *old_[6] = {};  // x, y, u, v, rho, tem
+  double *new_[6] = {};
+  double *rhs[4] = {};   // u, v, rho, tem (lssph_calculate_rhs only)
+  double *f_prev = nullptr;  // diffusion steady check
+  // virtual markers (marker k pairs with upload index n_int + k)
+  double *vm_x = nullptr, *vm_y = nullptr, *vm_nx = nullptr, *vm_ny = nullptr;
+  double *vm_u = nullptr, *vm_v = nullptr, *vm_pre = nullptr, *vm_tem = nullptr;
+  int32_t *vm_ptype = nullptr, *vm_inst = nullptr, *vm_cell = nullptr, *vm_wall = nullptr;
+  // reductions and counters
+  unsigned long long *umax = nullptr;    // [n_inst] bits of max |u|
+  unsigned long long *dfmax = nullptr;   // [n_inst] bits of max |f_prev - u|
+  int32_t *steady = nullptr;             // [n_inst]
+  int32_t *counters = nullptr;           // [kNumCounters]
+  unsigned long long *stats64 = nullptr;  // [kNumStats64]
+  // host-side staging for downloads (device scratch, upload order)
+  double *stage_f64 = nullptr;  // [8 * n_total]
+  int32_t *stage_i32 = nullptr;  // [n_total]
+  // tuning
+  int block = 128;
+  int list_len = 48;
+};
+
+// cells.hip
+void launch_fill_i32(hipStream_t s, int32_t *p, int32_t v, int n);
+void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n);
+int scan_tile_count(int n);
+void launch_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int32_t *tile_sums, int n);
+void launch_bin(hipStream_t s, Device &d, const Rec *rec, int fixed_key_mode);
+void launch_gather_rec(hipStream_t s, int n, const int32_t *perm, const Rec *src, Rec *dst);
+void launch_gather_i32(hipStream_t s, int n, const int32_t *perm, const int32_t *src, int32_t *dst);
+void launch_gather_f64(hipStream_t s, int n, const int32_t *perm, const double *src, double *dst);
+void launch_inverse_perm(hipStream_t s, int n, const int32_t *orig_of, int32_t *phys_of);
+
+// operators.hip
+// Stage codes of the fused LS type-A kernel.
+enum StageMode { kRhsOnly = 0, kRk2S1, kRk2S2, kRk4S1, kRk4S2, kRk4S3, kRk4S4 };
+void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode);
+void launch_umax(hipStream_t s, Device &d, int cur);
+void launch_pst(hipStream_t s, Device &d, int cur);
+void launch_ghost(hipStream_t s, Device &d, int cur);
+void launch_vm_init(hipStream_t s, Device &d, int cur);
+void launch_neighbor_count(hipStream_t s, Device &d, int cur, int32_t *counts_orig);
+void launch_neighbor_fill(hipStream_t s, Device &d, int cur, const int64_t *offsets_orig, int32_t *list);
+void launch_steady(hipStream_t s, Device &d, int cur);
+void launch_download(hipStream_t s, Device &d, int cur);
+void launch_download_rhs(hipStream_t s, Device &d);
+void launch_upload(hipStream_t s, Device &d, const double *stage, const int32_t *ptype, const int32_t *inst_of,
+                   int has_tem);
+
+}  // namespace lssph

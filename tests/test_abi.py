@@ -1,0 +1,84 @@
+"""The C-ABI library loads and exports every declared entry point (CPU only:
+no call here reaches the GPU)."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from lssph import config as lcfg
+from lssph import engine
+
+REPO = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (REPO / "include" / "lssph.h").read_text()
+    return sorted(set(re.findall(r"\b(lssph_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_what_the_binding_knows():
+    assert set(declared_symbols()) == set(engine.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = engine.load_library()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_fortran_binding_covers_every_symbol():
+    path = REPO / "ls-sph-benchmarks_amd" / "fortran" / "lssph_binding.f90"
+    if not path.exists():
+        pytest.skip("Fortran binding not written yet")
+    text = path.read_text().lower()
+    for name in declared_symbols():
+        assert f'name="{name}"' in text, name
+
+
+def _params(**kw):
+    cfg = lcfg.Config(target_problem=2, num_x=16, num_y=16, tg_a=2, tg_b=2, coe_h=1.55)
+    p = lcfg.init_param(cfg).to_c()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize("field,value", [("target_problem", 7), ("kernel_type", 0), ("sph_model", 5),
+                                         ("wall_model", 4), ("rk", 3), ("precision", 2), ("h", -1.0)])
+def test_create_rejects_bad_options_without_touching_the_gpu(field, value):
+    lib = engine.load_library()
+    p = _params(**{field: value})
+    ni = (ctypes.c_int32 * 1)(256)
+    ne = (ctypes.c_int32 * 1)(320)
+    h = ctypes.c_void_p()
+    rc = lib.lssph_create(ctypes.byref(p), 1, ni, ne, ctypes.byref(h))
+    assert rc == -1
+    assert lib.lssph_last_error(None)
+    assert not h.value
+
+
+def test_create_rejects_walls_in_a_periodic_box():
+    lib = engine.load_library()
+    p = _params(periodic=1)
+    ni = (ctypes.c_int32 * 1)(256)
+    ne = (ctypes.c_int32 * 1)(10)
+    h = ctypes.c_void_p()
+    assert lib.lssph_create(ctypes.byref(p), 1, ni, ne, ctypes.byref(h)) == -1
+    assert b"periodic" in lib.lssph_last_error(None)
+
+
+def test_mismatched_ensemble_options_rejected():
+    lib = engine.load_library()
+    a, b = _params(rk=4), _params(rk=2)
+    arr = (lcfg.Params * 2)(a, b)
+    ni = (ctypes.c_int32 * 2)(256, 256)
+    ne = (ctypes.c_int32 * 2)(320, 320)
+    h = ctypes.c_void_p()
+    assert lib.lssph_create(arr, 2, ni, ne, ctypes.byref(h)) == -1
+    assert b"differ" in lib.lssph_last_error(None)
+
+
+def test_params_struct_layout_matches_header():
+    # 29 doubles then 18 int32: 232 + 72 = 304 bytes, no padding
+    assert ctypes.sizeof(lcfg.Params) == 29 * 8 + 18 * 4
