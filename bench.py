@@ -135,6 +135,22 @@ def cpu_baseline(threads: int) -> dict | None:
         return None
 
 
+def copy_probe(nbytes: int) -> None:
+    """Three hipMemcpy device-to-device copies of nbytes: a transfer of known
+    size whose FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}), flush=True)
+
+
 def field_diff(a: dict, b: dict, n: int) -> dict:
     out = {}
     for k in ("x", "y", "u", "v", "rho", "pre"):
@@ -154,7 +170,12 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run 3 device-to-device copies of BYTES (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the domain-decomposition design is described in DESIGN.md); run with --gpus 1.")

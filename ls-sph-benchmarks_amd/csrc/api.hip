@@ -209,7 +209,12 @@ void enqueue_stage(lssph_handle *h, StageMode m) {
   Device &d = h->d;
   timed(h, kGrpLsa, [&] { launch_lsa(h->stream, d, h->cur, m); });
   h->cur ^= 1;
-  if (!is_diffusion(h)) timed(h, kGrpCells, [&] { launch_bin(h->stream, d, d.rec[h->cur], 0); });
+  // update_cell (skipped for the diffusion test, RK4_mod.f90:295-297, whose
+  // particles do not move: only the mirror's values are refreshed)
+  if (!is_diffusion(h))
+    timed(h, kGrpCells, [&] { launch_bin(h->stream, d, d.rec[h->cur], 0); });
+  else
+    timed(h, kGrpCells, [&] { launch_mirror(h->stream, d, d.rec[h->cur]); });
   if (d.n_ext) timed(h, kGrpGhost, [&] { launch_ghost(h->stream, d, h->cur); });
 }
 
@@ -296,11 +301,16 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   h->n_ext.assign(n_ext, n_ext + n_inst);
   Device &d = h->d;
   d.n_inst = n_inst;
-  int64_t ni = 0, ne = 0, nc = 0;
+  int64_t ni = 0, ne = 0, nc = 0, pb = 0;
   for (int k = 0; k < n_inst; ++k) {
     InstDev I = make_inst(params[k]);
     I.cell_off = (int32_t)nc;
     nc += (int64_t)I.nx * I.ny;
+    // physical order is sorted by cell, and cells are instance-major, so an
+    // instance owns the contiguous physical range [pbeg, pend)
+    I.pbeg = (int32_t)pb;
+    pb += (int64_t)n_int[k] + n_ext[k];
+    I.pend = (int32_t)pb;
     h->inst_host.push_back(I);
     ni += n_int[k];
     ne += n_ext[k];
@@ -316,7 +326,14 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   const lssph_params_t &a = params[0];
   d.opt = Options{a.target_problem, a.kernel_type, a.wall_model, a.rk, a.u_bc_top, a.u_bc_bottom,
                   a.u_bc_left, a.u_bc_right, a.tem_bc_top, a.tem_bc_bottom, a.tem_bc_left, a.tem_bc_right,
-                  a.target_problem == 4 ? 1 : 0, a.precision};
+                  a.target_problem == 4 ? 1 : 0, a.precision, 0, 0};
+  for (int k = 0; k < n_inst; ++k)
+    if (params[k].coe_delta_sph != 0.0) d.opt.delta = 1;
+  {
+    int64_t nmax = 1;
+    for (int k = 0; k < n_inst; ++k) nmax = std::max<int64_t>(nmax, (int64_t)n_int[k] + n_ext[k]);
+    d.red_blocks = (int)std::min<int64_t>(256, (nmax + 4095) / 4096);
+  }
   // upload index -> instance: fluid blocks, then wall blocks, in instance order
   h->inst_of.resize(d.n_total);
   {
@@ -338,6 +355,10 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   rc = rc ? rc : dalloc(h, &d.cell_count, d.n_cells);
   rc = rc ? rc : dalloc(h, &d.cell_start, d.n_cells + 1);
   rc = rc ? rc : dalloc(h, &d.items, n);
+  rc = rc ? rc : dalloc(h, &d.mirror, n);
+  rc = rc ? rc : dalloc(h, &d.sinfo, n);
+  rc = rc ? rc : dalloc(h, &d.slot_of, n);
+  rc = rc ? rc : dalloc(h, &d.fb_list, n);
   rc = rc ? rc : dalloc(h, &d.scan_tmp, scan_tile_count(d.n_cells) + 1);
   rc = rc ? rc : dalloc(h, &d.perm_tmp, n);
   for (int f = 0; f < 6 && !rc; ++f) rc = dalloc(h, &d.old_[f], n);

@@ -1,0 +1,304 @@
+// boundary.hip -- ghost-wall particles for gfx950 (ghost_mod.f90:43-78).
+//
+// One thread per virtual marker (the mirror of a wall particle across the
+// wall): interpolate the fluid values at the marker from its fluid
+// neighbors in the cell-sorted mirror (LS type B with q = 3 or 6, or CSPH),
+// then extrapolate them to the paired wall particle per wall type and
+// boundary condition.  The wall values are written to the mirror slot (read
+// by the next pair kernel) and to the physical record.
+#include "pair_common.hpp"
+
+namespace lssph {
+
+struct GhostArgs {
+  Rec *__restrict__ mir;   // fluid read, wall slots written
+  Rec *__restrict__ rec;   // physical records of the current buffer (walls written)
+  const int32_t *__restrict__ sinfo;
+  const int32_t *__restrict__ slot_of;
+  const int32_t *__restrict__ cstart;
+  const InstDev *__restrict__ inst;
+  const double *__restrict__ vm_x, *__restrict__ vm_y, *__restrict__ vm_nx, *__restrict__ vm_ny;
+  const int32_t *__restrict__ vm_ptype, *__restrict__ vm_inst, *__restrict__ vm_cell, *__restrict__ vm_wall;
+  double *vm_u, *vm_v, *vm_pre, *vm_tem;
+  Options opt;
+  int n_ext;
+};
+
+// Hydrostatic pressure of an isothermal wall side (extrapolate_VM_to_WL_mod
+// .f90:57-61, 105-109, 209-213, 250-254).
+__device__ __forceinline__ double wall_pre(const InstDev &I, bool has_t, int iso, double tw, double vpre,
+                                           double wall_y, double vm_y) {
+  if (has_t && iso == 1)
+    return vpre + I.rho_ref * I.alpha_ref * (tw - I.tem_ave) * I.gravity * (wall_y - vm_y);
+  return vpre;
+}
+
+// interpolate_LSSPH_B_mod.f90:24-178: a = [1, x/h, y/h (, x^2/2h^2, xy/h^2,
+// y^2/2h^2)], fluid neighbors only, value = constant term of the fit.
+template <int Q>
+__device__ __forceinline__ void interp_lssph_b(const GhostArgs &A, const InstDev &I, int k, int kt, bool has_t,
+                                               double (&val)[4]) {
+  double M[Q * (Q + 1) / 2];
+  double b[4][Q];
+#pragma unroll
+  for (int i = 0; i < Q * (Q + 1) / 2; ++i) M[i] = 0.0;
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int i = 0; i < Q; ++i) b[f][i] = 0.0;
+  const double mx = A.vm_x[k], my = A.vm_y[k];
+  const double inv_h = I.inv_h, inv_h2 = I.inv_h_sq;
+  int num = 0;
+  visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int) {
+    for (int t = t0; t < t1; ++t) {
+      if (info_ptype(A.sinfo[t]) != 0) continue;
+      const double2 xy = load_xy(A.mir, t);
+      const double xj = xy.x - mx, yj = xy.y - my;
+      const double r2 = sq_dist(xj, yj);
+      if (r2 > I.h_eff_sq) continue;
+      const Rec nb = A.mir[t];
+      const double w = I.w_norm * kshape_w<double>(kt, sqrt(r2) * I.inv_h);
+      ++num;
+      const double vw = nb.vol * w;
+      double a[Q];
+      a[0] = 1.0;
+      a[1] = xj * inv_h;
+      a[2] = yj * inv_h;
+      if constexpr (Q == 6) {
+        a[3] = xj * xj * 0.5 * inv_h2;
+        a[4] = xj * yj * inv_h2;
+        a[5] = yj * yj * 0.5 * inv_h2;
+      }
+      const double fv[4] = {nb.u, nb.v, nb.pre, has_t ? nb.tem : 0.0};
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const double vwa = vw * a[i];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) b[f][i] += vwa * fv[f];
+#pragma unroll
+        for (int r = 0; r <= i; ++r) M[i * (i + 1) / 2 + r] += vwa * a[r];
+      }
+    }
+  });
+  if (num >= Q) {
+    double Am[Q][Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+#pragma unroll
+      for (int r = 0; r <= i; ++r) {
+        Am[r][i] = M[i * (i + 1) / 2 + r];
+        Am[i][r] = Am[r][i];
+      }
+    lu_solve<Q, 4>(Am, b);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) val[f] = b[f][0];
+  } else {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) val[f] = 0.0;
+  }
+}
+
+// interpolate_CSPH_mod.f90:24-130: Shepard average of the fluid neighbors.
+__device__ __forceinline__ void interp_csph(const GhostArgs &A, const InstDev &I, int k, int kt, bool has_t,
+                                            double (&val)[4]) {
+  const double mx = A.vm_x[k], my = A.vm_y[k];
+  double su = 0, sv = 0, sp = 0, st = 0, sw = 0;
+  int num = 0;
+  visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int) {
+    for (int t = t0; t < t1; ++t) {
+      if (info_ptype(A.sinfo[t]) != 0) continue;
+      const double2 xy = load_xy(A.mir, t);
+      const double xij = mx - xy.x, yij = my - xy.y;
+      const double r2 = sq_dist(xij, yij);
+      if (r2 > I.h_eff_sq) continue;
+      const Rec nb = A.mir[t];
+      const double w = I.w_norm * kshape_w<double>(kt, sqrt(r2) * I.inv_h);
+      ++num;
+      const double vw = nb.vol * w;
+      sw = sw + vw;
+      su = su + vw * nb.u;
+      sv = sv + vw * nb.v;
+      sp = sp + vw * nb.pre;
+      if (has_t) st = st + vw * nb.tem;
+    }
+  });
+  if (num > 0 && sw > 1.0e-12) {
+    val[0] = su / sw;
+    val[1] = sv / sw;
+    val[2] = sp / sw;
+    val[3] = st / sw;
+  } else {
+    val[0] = val[1] = val[2] = val[3] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ghost(GhostArgs A) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.n_ext) return;
+  const Options &o = A.opt;
+  const InstDev &I = A.inst[A.vm_inst[k]];
+  const bool has_t = o.has_tem != 0;
+  double val[4];
+  if (o.wall_model == 1)
+    interp_csph(A, I, k, o.kernel_type, has_t, val);
+  else if (o.wall_model == 3)
+    interp_lssph_b<6>(A, I, k, o.kernel_type, has_t, val);
+  else
+    interp_lssph_b<3>(A, I, k, o.kernel_type, has_t, val);
+  A.vm_u[k] = val[0];
+  A.vm_v[k] = val[1];
+  A.vm_pre[k] = val[2];
+  if (has_t) A.vm_tem[k] = val[3];
+
+  const int pw = A.vm_wall[k];
+  Rec w = A.rec[pw];
+  const int t = A.vm_ptype[k];
+  const double vu = val[0], vv = val[1], vp = val[2], vt = val[3];
+  if (o.target_problem == 1) {
+    // extrapolate_diff_test_mod.f90:38-104
+    const double vx = A.vm_x[k], vy = A.vm_y[k];
+    if (t == 1)
+      w.u = (o.u_bc_bottom == 1) ? vu : 2.0 * (sin(kPi * (vx - I.wl_thick))) - vu;
+    else if (t == 2)
+      w.u = (o.u_bc_top == 1) ? vu : 2.0 * (-sin(kPi * (vx - I.wl_thick))) - vu;
+    else if (t == 3)
+      w.u = (o.u_bc_left == 1) ? vu + kPi * cos(kPi * (vy - I.wl_thick)) * (w.x - vx) : -vu;
+    else if (t == 4)
+      w.u = (o.u_bc_right == 1) ? vu + (-kPi) * cos(kPi * (vy - I.wl_thick)) * (w.x - vx) : -vu;
+    else
+      w.u = -vu;
+  } else {
+    // extrapolate_VM_to_WL_mod.f90:40-273
+    const double nx = A.vm_nx[k], ny = A.vm_ny[k];
+    if (t >= 1 && t <= 4) {
+      const int bc = (t == 1) ? o.u_bc_bottom : (t == 2) ? o.u_bc_top : (t == 3) ? o.u_bc_left : o.u_bc_right;
+      if (bc == 1) {
+        const double un = vu * nx + vv * ny;
+        w.u = vu - 2.0 * un * nx;
+        w.v = vv - 2.0 * un * ny;
+      } else if (bc == 2) {
+        w.u = -vu;
+        w.v = -vv;
+      } else if (bc == 3 && t == 2) {
+        w.u = 2.0 * I.u_top - vu;
+        w.v = -vv;
+      }
+    }
+    const double vmy = A.vm_y[k];
+    switch (t) {
+      case 1:
+        w.pre = wall_pre(I, has_t, o.tem_bc_bottom, I.tem_bottom, vp, w.y, vmy);
+        if (has_t) w.tem = (o.tem_bc_bottom == 1) ? 2.0 * I.tem_bottom - vt : vt;
+        break;
+      case 2:
+        w.pre = wall_pre(I, has_t, o.tem_bc_top, I.tem_top, vp, w.y, vmy);
+        if (has_t) w.tem = (o.tem_bc_top == 1) ? 2.0 * I.tem_top - vt : vt;
+        break;
+      case 3:
+        w.pre = vp;
+        if (has_t) w.tem = (o.tem_bc_left == 1) ? 2.0 * I.tem_left - vt : vt;
+        break;
+      case 4:
+        w.pre = vp;
+        if (has_t) w.tem = (o.tem_bc_right == 1) ? 2.0 * I.tem_right - vt : vt;
+        break;
+      case 5:
+      case 6:
+        w.u = -vu;
+        w.v = -vv;
+        w.pre = wall_pre(I, has_t, o.tem_bc_bottom, I.tem_bottom, vp, w.y, vmy);
+        if (has_t) w.tem = (o.tem_bc_bottom == 1) ? 2.0 * I.tem_bottom - vt : vt;
+        break;
+      default:  // 7, 8
+        if (o.u_bc_top == 3) {
+          w.u = 2.0 * I.u_top - vu;
+          w.v = -vv;
+        } else {
+          w.u = -vu;
+          w.v = -vv;
+        }
+        w.pre = wall_pre(I, has_t, o.tem_bc_top, I.tem_top, vp, w.y, vmy);
+        if (has_t) w.tem = (o.tem_bc_top == 1) ? 2.0 * I.tem_top - vt : vt;
+        break;
+    }
+  }
+  A.rec[pw] = w;
+  A.mir[A.slot_of[pw]] = w;
+}
+
+// Virtual markers from the wall positions (init_virtual_markers_mod.f90:
+// 61-137) and their fixed cells (init_cell_mod.f90:166-176).
+__global__ void k_vm_init(int n_int, int n_ext, const Rec *__restrict__ rec, const int32_t *__restrict__ info,
+                          const int32_t *__restrict__ phys_of, const InstDev *__restrict__ inst, double *vm_x,
+                          double *vm_y, double *vm_nx, double *vm_ny, int32_t *vm_ptype, int32_t *vm_inst,
+                          int32_t *vm_cell, int32_t *vm_wall, int32_t *escaped) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_ext) return;
+  const int pw = phys_of[n_int + k];
+  const int32_t inf = info[pw];
+  const int t = info_ptype(inf), ii = info_inst(inf);
+  const InstDev &I = inst[ii];
+  const double wx = rec[pw].x, wy = rec[pw].y;
+  const double r2 = 1.0 / sqrt(2.0);
+  double x = wx, y = wy, nx = 0.0, ny = 0.0;
+  switch (t) {
+    case 1: y = 2.0 * I.y_min - wy; ny = -1.0; break;
+    case 2: y = 2.0 * I.y_max - wy; ny = 1.0; break;
+    case 3: x = 2.0 * I.x_min - wx; nx = -1.0; break;
+    case 4: x = 2.0 * I.x_max - wx; nx = 1.0; break;
+    case 5: x = 2.0 * I.x_min - wx; y = 2.0 * I.y_min - wy; nx = -r2; ny = -r2; break;
+    case 6: x = 2.0 * I.x_max - wx; y = 2.0 * I.y_min - wy; nx = r2; ny = -r2; break;
+    case 7: x = 2.0 * I.x_min - wx; y = 2.0 * I.y_max - wy; nx = -r2; ny = r2; break;
+    default: x = 2.0 * I.x_max - wx; y = 2.0 * I.y_max - wy; nx = r2; ny = r2; break;
+  }
+  int c = fixed_cell(I, x, y);
+  if (t < 1 || t > 8 || c < 0) {
+    atomicAdd(escaped, 1);
+    c = 0;
+  }
+  vm_x[k] = x;
+  vm_y[k] = y;
+  vm_nx[k] = nx;
+  vm_ny[k] = ny;
+  vm_ptype[k] = t;
+  vm_inst[k] = ii;
+  vm_cell[k] = I.cell_off + c;
+  vm_wall[k] = pw;
+}
+
+static inline int blocks(int n, int b) { return (n + b - 1) / b; }
+
+void launch_ghost(hipStream_t s, Device &d, int cur) {
+  if (d.n_ext == 0) return;
+  GhostArgs a;
+  a.mir = d.mirror;
+  a.rec = d.rec[cur];
+  a.sinfo = d.sinfo;
+  a.slot_of = d.slot_of;
+  a.cstart = d.cell_start;
+  a.inst = d.inst;
+  a.vm_x = d.vm_x;
+  a.vm_y = d.vm_y;
+  a.vm_nx = d.vm_nx;
+  a.vm_ny = d.vm_ny;
+  a.vm_ptype = d.vm_ptype;
+  a.vm_inst = d.vm_inst;
+  a.vm_cell = d.vm_cell;
+  a.vm_wall = d.vm_wall;
+  a.vm_u = d.vm_u;
+  a.vm_v = d.vm_v;
+  a.vm_pre = d.vm_pre;
+  a.vm_tem = d.vm_tem;
+  a.opt = d.opt;
+  a.n_ext = d.n_ext;
+  hipLaunchKernelGGL(k_ghost, dim3(blocks(d.n_ext, 128)), dim3(128), 0, s, a);
+}
+
+void launch_vm_init(hipStream_t s, Device &d, int cur) {
+  if (d.n_ext == 0) return;
+  hipLaunchKernelGGL(k_vm_init, dim3(blocks(d.n_ext, 256)), dim3(256), 0, s, d.n_int, d.n_ext, d.rec[cur],
+                     d.info, d.phys_of, d.inst, d.vm_x, d.vm_y, d.vm_nx, d.vm_ny, d.vm_ptype, d.vm_inst,
+                     d.vm_cell, d.vm_wall, d.counters + kEscaped);
+}
+
+}  // namespace lssph

@@ -6,7 +6,7 @@
 // list in ascending index), so pair sums are accumulated in the reference's
 // order.  Passes: key + count (atomics on ~10 particles per cell) -> three
 // kernel exclusive scan -> scatter -> per-cell insertion sort.
-#include "internal.hpp"
+#include "pair_common.hpp"
 
 namespace lssph {
 
@@ -20,67 +20,59 @@ __global__ void k_fill_u64(unsigned long long *__restrict__ p, unsigned long lon
   if (i < n) p[i] = v;
 }
 
-// Cell of a fluid particle: update_cell_mod.f90:53-55 (ceiling(x*inv_h_eff),
-// 1-based) for the closed box; floor(x * cells / L) for a periodic box.
-// Returns -1 outside the grid.
-__device__ __forceinline__ int fluid_cell(const InstDev &I, double x, double y) {
-  int cx, cy;
-  if (I.periodic) {
-    cx = (int)floor(x * I.inv_cw_x);
-    cy = (int)floor(y * I.inv_cw_y);
-    cx = (cx == I.nx) ? I.nx - 1 : cx;
-    cy = (cy == I.ny) ? I.ny - 1 : cy;
-  } else {
-    cx = (int)ceil(x * I.inv_h_eff) - 1;
-    cy = (int)ceil(y * I.inv_h_eff) - 1;
-  }
-  if (cx < 0 || cx >= I.nx || cy < 0 || cy >= I.ny) return -1;
-  return cx + cy * I.nx;
+// Lanes of a wavefront holding equal keys in consecutive lanes form runs
+// (particles are kept roughly in cell order, so runs average several
+// particles); one lane per run does the atomic for the whole run.
+struct Run {
+  int head_lane;  // first lane of my run
+  int length;     // lanes in my run (valid on the head lane)
+  bool head;
+};
+
+__device__ __forceinline__ Run lane_runs(int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1);
+  const bool head = (lane == 0) || (prev != key);
+  const unsigned long long hm = __ballot(head);
+  const unsigned long long above = (lane == 63) ? 0ull : (hm >> (lane + 1)) << (lane + 1);
+  const int next = above ? __ffsll((long long)above) - 1 : 64;
+  const unsigned long long upto = (lane == 63) ? hm : (hm & ((2ull << lane) - 1ull));
+  Run r;
+  r.head = head;
+  r.length = next - lane;
+  r.head_lane = 63 - __clzll((long long)upto);
+  return r;
 }
 
-// Cell of a wall particle or virtual marker: init_cell_mod.f90:76-78 and
-// :170-172 divide by h_eff instead of multiplying by its inverse.
-__device__ __forceinline__ int fixed_cell(const InstDev &I, double x, double y) {
-  int cx = (int)ceil(x / I.h_eff) - 1;
-  int cy = (int)ceil(y / I.h_eff) - 1;
-  if (cx < 0 || cx >= I.nx || cy < 0 || cy >= I.ny) return -1;
-  return cx + cy * I.nx;
-}
-
-// Keys of every particle and per-cell counts.  Walls keep the key set at
-// init (fixed_key_mode = 1 computes it).  Escaped particles are counted and
-// clamped into the grid so that memory stays in bounds.
-__global__ void k_bin_count(int n, const Rec *__restrict__ rec, const int32_t *__restrict__ info,
-                            const InstDev *__restrict__ inst, int32_t *__restrict__ key,
-                            int32_t *__restrict__ count, int32_t *__restrict__ escaped,
-                            int fixed_key_mode) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int32_t inf = info[p];
-  const InstDev &I = inst[info_inst(inf)];
-  int k;
-  if (info_ptype(inf) == 0) {
-    const double2 xy = *reinterpret_cast<const double2 *>(&rec[p].x);
-    int c = fluid_cell(I, xy.x, xy.y);
-    if (c < 0) {
-      atomicAdd(escaped, 1);
-      c = 0;
+// Keys of every particle and per-cell counts.  Fluid keys follow the
+// particle positions; wall keys are fixed at init (fixed_key_mode = 1
+// computes them).  Escaped particles are counted and clamped into the grid
+// so that memory stays in bounds.
+__global__ void __launch_bounds__(256) k_bin_count(int n, const Rec *__restrict__ rec,
+                                                   const int32_t *__restrict__ info,
+                                                   const InstDev *__restrict__ inst, int32_t *__restrict__ key,
+                                                   int32_t *__restrict__ count, int32_t *__restrict__ escaped,
+                                                   int fixed_key_mode) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = -1 - (threadIdx.x & 63);  // distinct sentinel per idle lane
+  if (p < n) {
+    const int32_t inf = info[p];
+    const InstDev &I = inst[info_inst(inf)];
+    if (info_ptype(inf) == 0 || fixed_key_mode) {
+      const double2 xy = *reinterpret_cast<const double2 *>(&rec[p].x);
+      int c = info_ptype(inf) == 0 ? fluid_cell(I, xy.x, xy.y) : fixed_cell(I, xy.x, xy.y);
+      if (c < 0) {
+        atomicAdd(escaped, 1);
+        c = 0;
+      }
+      k = I.cell_off + c;
+      key[p] = k;
+    } else {
+      k = key[p];
     }
-    k = I.cell_off + c;
-    key[p] = k;
-  } else if (fixed_key_mode) {
-    const double2 xy = *reinterpret_cast<const double2 *>(&rec[p].x);
-    int c = fixed_cell(I, xy.x, xy.y);
-    if (c < 0) {
-      atomicAdd(escaped, 1);
-      c = 0;
-    }
-    k = I.cell_off + c;
-    key[p] = k;
-  } else {
-    k = key[p];
   }
-  atomicAdd(&count[k], 1);
+  const Run r = lane_runs(k);
+  if (r.head && k >= 0) atomicAdd(&count[k], r.length);
 }
 
 // --- exclusive scan of the cell counts ------------------------------------
@@ -148,23 +140,52 @@ __global__ void k_scan_add(int32_t *__restrict__ out, const int32_t *__restrict_
   if (i < n) out[i] += tile_sums[i / kScanTile];
 }
 
-// Scatter: the slot inside the cell comes from decrementing the count, which
-// leaves count[] zero for the next build.
-__global__ void k_scatter(int n, const int32_t *__restrict__ key, const int32_t *__restrict__ start,
-                          int32_t *__restrict__ count, int32_t *__restrict__ items) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int k = key[p];
-  const int slot = atomicSub(&count[k], 1) - 1;
-  items[start[k] + slot] = p;
+// Scatter: a run reserves its slots by decrementing the cell count once,
+// which also leaves count[] zero for the next build.  The order inside a
+// cell is fixed afterwards by k_cell_sort.
+__global__ void __launch_bounds__(256) k_scatter(int n, const int32_t *__restrict__ key,
+                                                 const int32_t *__restrict__ start, int32_t *__restrict__ count,
+                                                 int32_t *__restrict__ items) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int k = (p < n) ? key[p] : -1 - lane;
+  const Run r = lane_runs(k);
+  int base = 0;
+  if (r.head && k >= 0) base = atomicSub(&count[k], r.length) - r.length;
+  base = __shfl(base, r.head_lane);
+  if (k >= 0) items[start[k] + base + (lane - r.head_lane)] = p;
 }
 
-// Per-cell insertion sort of the slice by original particle index.
-__global__ void k_cell_sort(int ncells, const int32_t *__restrict__ start, int32_t *__restrict__ items,
-                            const int32_t *__restrict__ orig_of, int32_t *__restrict__ max_count) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+// Per-cell sort of the slice by original particle index (insertion sort on
+// the cell's <= kSortLds entries staged in LDS; larger cells sort in place
+// in global memory).  Also records the largest cell count.
+constexpr int kSortThreads = 256;
+constexpr int kSortLds = 16;
+
+__global__ void __launch_bounds__(kSortThreads)
+k_cell_sort(int ncells, const int32_t *__restrict__ start, int32_t *__restrict__ items,
+            const int32_t *__restrict__ orig_of, int32_t *__restrict__ max_count) {
+  __shared__ int32_t sh_item[kSortLds][kSortThreads];
+  __shared__ int32_t sh_key[kSortLds][kSortThreads];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x;
   const int s = (c < ncells) ? start[c] : 0, e = (c < ncells) ? start[c + 1] : 0;
-  if (e - s > 1) {
+  const int m = e - s;
+  if (m > 1 && m <= kSortLds) {
+    for (int i = 0; i < m; ++i) {
+      const int it = items[s + i];
+      const int ko = orig_of[it];
+      int j = i - 1;
+      while (j >= 0 && sh_key[j][tid] > ko) {
+        sh_key[j + 1][tid] = sh_key[j][tid];
+        sh_item[j + 1][tid] = sh_item[j][tid];
+        --j;
+      }
+      sh_key[j + 1][tid] = ko;
+      sh_item[j + 1][tid] = it;
+    }
+    for (int i = 0; i < m; ++i) items[s + i] = sh_item[i][tid];
+  } else if (m > kSortLds) {
     for (int i = s + 1; i < e; ++i) {
       const int it = items[i];
       const int ko = orig_of[it];
@@ -176,10 +197,23 @@ __global__ void k_cell_sort(int ncells, const int32_t *__restrict__ start, int32
       items[j + 1] = it;
     }
   }
-  int m = e - s;
+  int mm = m;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
-  if ((threadIdx.x & 63) == 0) atomicMax(max_count, m);
+  for (int o = 32; o > 0; o >>= 1) mm = max(mm, __shfl_xor(mm, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(max_count, mm);
+}
+
+// Cell-sorted mirror of the records: mirror[t] = rec[items[t]], the info of
+// each slot, and the slot of every physical particle.
+__global__ void k_mirror(int n, const int32_t *__restrict__ items, const Rec *__restrict__ rec,
+                         const int32_t *__restrict__ info, Rec *__restrict__ mirror, int32_t *__restrict__ sinfo,
+                         int32_t *__restrict__ slot_of) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int p = items[t];
+  mirror[t] = rec[p];
+  sinfo[t] = info[p];
+  slot_of[p] = t;
 }
 
 // --- permutations (physical re-sort) --------------------------------------
@@ -233,8 +267,14 @@ void launch_bin(hipStream_t s, Device &d, const Rec *rec, int fixed_key_mode) {
   launch_exclusive_scan(s, d.cell_count, d.cell_start, d.scan_tmp, d.n_cells);
   hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, s, n, d.key, d.cell_start,
                      d.cell_count, d.items);
-  hipLaunchKernelGGL(k_cell_sort, dim3(blocks(d.n_cells, 256)), dim3(256), 0, s, d.n_cells, d.cell_start,
-                     d.items, d.orig_of, d.counters + kMaxCell);
+  hipLaunchKernelGGL(k_cell_sort, dim3(blocks(d.n_cells, kSortThreads)), dim3(kSortThreads), 0, s, d.n_cells,
+                     d.cell_start, d.items, d.orig_of, d.counters + kMaxCell);
+  launch_mirror(s, d, rec);
+}
+
+void launch_mirror(hipStream_t s, Device &d, const Rec *rec) {
+  hipLaunchKernelGGL(k_mirror, dim3(blocks(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.items, rec, d.info,
+                     d.mirror, d.sinfo, d.slot_of);
 }
 
 void launch_gather_rec(hipStream_t s, int n, const int32_t *perm, const Rec *src, Rec *dst) {

@@ -53,6 +53,8 @@ struct Options {
   int32_t u_bc_top, u_bc_bottom, u_bc_left, u_bc_right;
   int32_t tem_bc_top, tem_bc_bottom, tem_bc_left, tem_bc_right;
   int32_t has_tem, precision;
+  int32_t delta;  // any instance with coe_delta_sph != 0
+  int32_t pad;
 };
 
 // ---------------------------------------------------------------------------

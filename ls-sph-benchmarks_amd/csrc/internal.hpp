@@ -10,7 +10,7 @@
 namespace lssph {
 
 // Indices into Device::counters (int32) and Device::stats64 (uint64).
-enum { kEscaped = 0, kMaxCell = 1, kMaxNbr = 2, kNumCounters = 4 };
+enum { kEscaped = 0, kMaxCell = 1, kMaxNbr = 2, kFbCount = 3, kNumCounters = 4 };
 enum { kFallbacks = 0, kOverflows = 1, kNumStats64 = 2 };
 
 // Raw device pointers and sizes; passed to the launchers.
@@ -25,7 +25,11 @@ struct Device {
   int32_t *phys_of = nullptr;       // [n_total] upload index -> physical
   int32_t *cell_count = nullptr;    // [n_cells]
   int32_t *cell_start = nullptr;    // [n_cells + 1]
-  int32_t *items = nullptr;         // [n_total] physical ids sorted by cell
+  int32_t *items = nullptr;         // [n_total] slot -> physical id (cells in order)
+  Rec *mirror = nullptr;            // [n_total] records in slot order
+  int32_t *sinfo = nullptr;         // [n_total] info in slot order
+  int32_t *slot_of = nullptr;       // [n_total] physical id -> slot
+  int32_t *fb_list = nullptr;       // [n_total] slots queued for the FP64 fallback
   int32_t *scan_tmp = nullptr;      // scan tile sums
   int32_t *perm_tmp = nullptr;      // [n_total]
   // RK work arrays (physical order, fluid entries used)
@@ -48,7 +52,8 @@ struct Device {
   int32_t *stage_i32 = nullptr;  // [n_total]
   // tuning
   int block = 128;
-  int list_len = 48;
+  int list_len = 40;
+  int red_blocks = 256;  // blocks per instance of the max reductions
 };
 
 // cells.hip
@@ -57,6 +62,7 @@ void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v,
 int scan_tile_count(int n);
 void launch_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int32_t *tile_sums, int n);
 void launch_bin(hipStream_t s, Device &d, const Rec *rec, int fixed_key_mode);
+void launch_mirror(hipStream_t s, Device &d, const Rec *rec);
 void launch_gather_rec(hipStream_t s, int n, const int32_t *perm, const Rec *src, Rec *dst);
 void launch_gather_i32(hipStream_t s, int n, const int32_t *perm, const int32_t *src, int32_t *dst);
 void launch_gather_f64(hipStream_t s, int n, const int32_t *perm, const double *src, double *dst);
@@ -67,6 +73,7 @@ void launch_inverse_perm(hipStream_t s, int n, const int32_t *orig_of, int32_t *
 enum StageMode { kRhsOnly = 0, kRk2S1, kRk2S2, kRk4S1, kRk4S2, kRk4S3, kRk4S4 };
 void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode);
 void launch_umax(hipStream_t s, Device &d, int cur);
+void launch_dfmax(hipStream_t s, Device &d, int cur);
 void launch_pst(hipStream_t s, Device &d, int cur);
 void launch_ghost(hipStream_t s, Device &d, int cur);
 void launch_vm_init(hipStream_t s, Device &d, int cur);

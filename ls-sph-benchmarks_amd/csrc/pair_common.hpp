@@ -1,0 +1,168 @@
+// pair_common.hpp -- neighbor traversal shared by the pair kernels.
+//
+// Neighbor data comes from the cell-sorted mirror: after every cell build
+// the particle records are gathered into slot order (cells consecutive, each
+// cell's particles in the reference's list order, update_cell_mod.f90:
+// 57-92), so the 3x3 stencil of a particle is at most three contiguous slot
+// ranges (four to six when a periodic row wraps) and lanes of a wavefront
+// read the same or adjacent cache lines.
+#pragma once
+
+#include "internal.hpp"
+
+namespace lssph {
+
+// LDS list entries: slot in the low bits, periodic image code above.
+constexpr int kSlotBits = 27;
+constexpr int kSlotMask = (1 << kSlotBits) - 1;
+
+// Image shift of a wrapped periodic cell: code 1 -> -len, 2 -> +len.
+__device__ __forceinline__ double shift_of(int code, double len) {
+  return code == 1 ? -len : (code == 2 ? len : 0.0);
+}
+
+// r^2 without FMA contraction, as the reference forms it (LSSPH_A_mod.f90:
+// 123), so the acceptance test r^2 <= h_eff^2 decides exactly as it does.
+__device__ __forceinline__ double sq_dist(double dx, double dy) {
+#pragma clang fp contract(off)
+  return dx * dx + dy * dy;
+}
+
+__device__ __forceinline__ double wrap_pos(double x, double len, bool periodic) {
+  if (!periodic) return x;
+  if (x >= len) return x - len;
+  if (x < 0.0) return x + len;
+  return x;
+}
+
+__device__ __forceinline__ double2 load_xy(const Rec *__restrict__ r, int j) {
+  return *reinterpret_cast<const double2 *>(&r[j].x);
+}
+
+// Visits the slot ranges of the 3x3 stencil around local cell `lc` in the
+// reference's order (rows dy = -1..1, within a row dx = -1..1), passing the
+// periodic image code of each range (x code in bits 0-1, y code in 2-3).
+template <typename F>
+__device__ __forceinline__ void visit_cells(const InstDev &I, int lc, const int32_t *__restrict__ cstart,
+                                            F &&f) {
+  const int nx = I.nx, ny = I.ny;
+  const int cx = lc % nx, cy = lc / nx;
+#pragma unroll
+  for (int dy = -1; dy <= 1; ++dy) {
+    int yy = cy + dy, ycode = 0;
+    if (I.periodic) {
+      if (yy < 0) {
+        yy += ny;
+        ycode = 1;
+      } else if (yy >= ny) {
+        yy -= ny;
+        ycode = 2;
+      }
+    } else if (yy < 0 || yy >= ny) {
+      continue;
+    }
+    const int row = I.cell_off + yy * nx;
+    const int yc = ycode << 2;
+    if (!I.periodic) {
+      const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < nx - 1 ? cx + 1 : nx - 1;
+      f(cstart[row + x0], cstart[row + x1 + 1], yc);
+    } else if (cx == 0) {
+      f(cstart[row + nx - 1], cstart[row + nx], yc | 1);
+      f(cstart[row], cstart[row + 2], yc);
+    } else if (cx == nx - 1) {
+      f(cstart[row + cx - 1], cstart[row + cx + 1], yc);
+      f(cstart[row], cstart[row + 1], yc | 2);
+    } else {
+      f(cstart[row + cx - 1], cstart[row + cx + 2], yc);
+    }
+  }
+}
+
+// Candidate scan of slot s: fills the LDS list (slot | code << kSlotBits),
+// returns the neighbor count.  fluid_only skips non-fluid candidates (the
+// ghost markers see only fluid, interpolate_LSSPH_B_mod.f90:95).
+__device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, const int32_t *__restrict__ sinfo,
+                                               const int32_t *__restrict__ cstart, const InstDev &I, int lc,
+                                               int s, double xi, double yi, int32_t *lds, int list_len, int B,
+                                               int tid, bool fluid_only) {
+  int cnt = 0;
+  visit_cells(I, lc, cstart, [&](int t0, int t1, int code) {
+    const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
+    for (int t = t0; t < t1; ++t) {
+      if (t == s) continue;
+      if (fluid_only && info_ptype(sinfo[t]) != 0) continue;
+      const double2 xy = load_xy(mir, t);
+      const double dx = (xy.x + sx) - xi, dy = (xy.y + sy) - yi;
+      if (sq_dist(dx, dy) <= I.h_eff_sq) {
+        if (cnt < list_len) lds[cnt * B + tid] = t | (code << kSlotBits);
+        ++cnt;
+      }
+    }
+  });
+  return cnt;
+}
+
+// Calls f(nb_record, dx, dy) for every accepted neighbor of slot s in
+// summation order: from the LDS list, or (overflow, or list_len == 0) by a
+// direct walk over the candidates.
+template <typename F>
+__device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
+                                               const InstDev &I, int lc, int s, const Rec &me,
+                                               const int32_t *lds, int cnt, int list_len, int B, int tid,
+                                               F &&f) {
+  if (cnt <= list_len) {
+    for (int k = 0; k < cnt; ++k) {
+      const int e = lds[k * B + tid];
+      const int t = e & kSlotMask, code = e >> kSlotBits;
+      const Rec nb = mir[t];
+      const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
+      const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
+      f(nb, dx, dy);
+    }
+  } else {
+    visit_cells(I, lc, cstart, [&](int t0, int t1, int code) {
+      const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
+      for (int t = t0; t < t1; ++t) {
+        if (t == s) continue;
+        const double2 xy = load_xy(mir, t);
+        const double dx = (xy.x + sx) - me.x, dy = (xy.y + sy) - me.y;
+        if (sq_dist(dx, dy) <= I.h_eff_sq) f(mir[t], dx, dy);
+      }
+    });
+  }
+}
+
+// Local cell of a fluid position: update_cell_mod.f90:53-55 (ceiling of
+// x * inv_h_eff, 1-based) in the closed box, floor(x * cells / L) in a
+// periodic one.  Returns -1 outside the grid.
+__device__ __forceinline__ int fluid_cell(const InstDev &I, double x, double y) {
+  int cx, cy;
+  if (I.periodic) {
+    cx = (int)floor(x * I.inv_cw_x);
+    cy = (int)floor(y * I.inv_cw_y);
+    cx = (cx == I.nx) ? I.nx - 1 : cx;
+    cy = (cy == I.ny) ? I.ny - 1 : cy;
+  } else {
+    cx = (int)ceil(x * I.inv_h_eff) - 1;
+    cy = (int)ceil(y * I.inv_h_eff) - 1;
+  }
+  if (cx < 0 || cx >= I.nx || cy < 0 || cy >= I.ny) return -1;
+  return cx + cy * I.nx;
+}
+
+// Same, clamped into the grid (used after binning has flagged escapes).
+__device__ __forceinline__ int local_cell(const InstDev &I, double x, double y) {
+  const int c = fluid_cell(I, x, y);
+  return c < 0 ? 0 : c;
+}
+
+// Cell of a wall particle or virtual marker: init_cell_mod.f90:76-78 and
+// :170-172 divide by h_eff instead of multiplying by its inverse.
+__device__ __forceinline__ int fixed_cell(const InstDev &I, double x, double y) {
+  const int cx = (int)ceil(x / I.h_eff) - 1;
+  const int cy = (int)ceil(y / I.h_eff) - 1;
+  if (cx < 0 || cx >= I.nx || cy < 0 || cy >= I.ny) return -1;
+  return cx + cy * I.nx;
+}
+
+}  // namespace lssph
