@@ -208,7 +208,7 @@ k_cell_sort(int ncells, const int32_t *__restrict__ start, int32_t *__restrict__
 __global__ void k_mirror(int n, const int32_t *__restrict__ items, const Rec *__restrict__ rec,
                          const int32_t *__restrict__ info, Rec *__restrict__ mirror, int32_t *__restrict__ sinfo,
                          int32_t *__restrict__ slot_of) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int p = items[t];
   mirror[t] = rec[p];
@@ -273,7 +273,7 @@ void launch_bin(hipStream_t s, Device &d, const Rec *rec, int fixed_key_mode) {
 }
 
 void launch_mirror(hipStream_t s, Device &d, const Rec *rec) {
-  hipLaunchKernelGGL(k_mirror, dim3(blocks(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.items, rec, d.info,
+  hipLaunchKernelGGL(k_mirror, dim3(xcd_grid(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.items, rec, d.info,
                      d.mirror, d.sinfo, d.slot_of);
 }
 

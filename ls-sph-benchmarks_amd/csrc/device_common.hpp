@@ -61,8 +61,11 @@ struct Options {
 // Kernel shapes (kernel_functions_mod.f90:26-146) as functions of q = r/h
 // without the normalisation factor.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T kshape_w(int kt, T q) {
+// KT > 0 fixes the kernel at compile time (the runtime kt is then ignored);
+// KT = 0 selects it at run time.
+template <typename T, int KT = 0>
+__device__ __forceinline__ T kshape_w(int kt_rt, T q) {
+  const int kt = KT ? KT : kt_rt;
   const T one = T(1), half = T(0.5), two = T(2);
   if (kt == 3) {
     if (q > two) return T(0);
@@ -96,8 +99,9 @@ __device__ __forceinline__ T kshape_w(int kt, T q) {
   return (t4 * t4) * (one + T(4) * q + T(25) * (q * q) / T(4) + T(4) * (q * q * q));
 }
 
-template <typename T>
-__device__ __forceinline__ T kshape_dw(int kt, T q) {
+template <typename T, int KT = 0>
+__device__ __forceinline__ T kshape_dw(int kt_rt, T q) {
+  const int kt = KT ? KT : kt_rt;
   const T one = T(1), half = T(0.5), two = T(2);
   if (kt == 3) {
     if (q > two) return T(0);
@@ -215,6 +219,23 @@ __device__ __forceinline__ int lu_solve(T (&a)[N][N], T (&b)[NR][N]) {
   int info = lu_factor<N, NR>(a, piv);
   if (info == 0) lu_apply<N, NR>(a, piv, b);
   return info;
+}
+
+// XCD-aware block index: the hardware deals workgroups round-robin to the 8
+// XCDs (block b runs on XCD b % 8), each with its own L2.  Remapping gives
+// XCD k the contiguous logical blocks [k * G/8, (k+1) * G/8), so the slot
+// ranges whose neighbor data overlap share one L2.  The grid must be a
+// multiple of 8 blocks; logical blocks past the work return early.
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ int xcd_block() {
+  const int per = gridDim.x / kNumXcd;
+  return (blockIdx.x % kNumXcd) * per + blockIdx.x / kNumXcd;
+}
+
+__host__ __device__ inline int xcd_grid(int n, int block) {
+  const int g = (n + block - 1) / block;
+  return (g + kNumXcd - 1) / kNumXcd * kNumXcd;
 }
 
 // Atomic max of a non-negative double through its bit pattern (IEEE order

@@ -205,7 +205,7 @@ __device__ __forceinline__ void scale_derivs(const double (&X)[NF][5], const Ins
 
 // FP64 assembly + DGESV-semantics solve; d[f] = (d_x, d_y, d_xx, d_yy, d_xy)
 // (LSSPH_A_mod.f90:199-240).
-template <int CLS, bool DELTA>
+template <int CLS, bool DELTA, int KT>
 __device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
                                          const int32_t *lds, int cnt, int list_len, int B, int tid,
                                          double (&d)[Layout<CLS, DELTA>::NF][5]) {
@@ -216,7 +216,7 @@ __device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int
   walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, list_len, B, tid,
                  [&](const Rec &nb, double dx, double dy) {
                    const double r = sqrt(sq_dist(dx, dy));
-                   const double w = I.w_norm * kshape_w<double>(kt, r * I.inv_h);
+                   const double w = I.w_norm * kshape_w<double, KT>(kt, r * I.inv_h);
                    double a[5];
                    basis5(dx, dy, I.inv_h, I.inv_h2, a);
                    double df[NF];
@@ -230,7 +230,7 @@ __device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int
 }
 
 // Mixed-precision assembly + FP64 LDL^T with refinement; false -> fallback.
-template <int CLS, bool DELTA>
+template <int CLS, bool DELTA, int KT>
 __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
                                          const int32_t *lds, int cnt, int list_len, int B, int tid,
                                          double (&d)[Layout<CLS, DELTA>::NF][5]) {
@@ -242,7 +242,7 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
                  [&](const Rec &nb, double dx, double dy) {
                    const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
                    const float q = sqrtf(xs * xs + ys * ys);
-                   const float vw = float(nb.vol * I.w_norm) * kshape_w<float>(kt, q);
+                   const float vw = float(nb.vol * I.w_norm) * kshape_w<float, KT>(kt, q);
                    float a[5];
                    basis5s(xs, ys, a);
                    float df[NF];
@@ -390,12 +390,12 @@ __device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p
   A.rout[p] = out;
 }
 
-template <int CLS, bool MIXED, bool DELTA>
+template <int CLS, bool MIXED, bool DELTA, int KT>
 __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   constexpr int NF = Layout<CLS, DELTA>::NF;
   extern __shared__ int32_t lds[];
   const int tid = threadIdx.x, B = blockDim.x;
-  const int s = blockIdx.x * B + tid;
+  const int s = xcd_block() * B + tid;
   if (s >= A.n) return;
   const int32_t inf = A.sinfo[s];
   if (info_ptype(inf) != 0) return;
@@ -414,20 +414,20 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
     for (int i = 0; i < 5; ++i) d[f][i] = 0.0;
   if (cnt >= 5) {  // LSSPH_A_mod.f90:208 (num_neighbors >= q)
     if constexpr (MIXED) {
-      if (!derivs32<CLS, DELTA>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d)) {
+      if (!derivs32<CLS, DELTA, KT>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d)) {
         const int k = atomicAdd(A.fb_count, 1);
         A.fb_list[k] = s;
         return;
       }
     } else {
-      derivs64<CLS, DELTA>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d);
+      derivs64<CLS, DELTA, KT>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d);
     }
   }
   finish<CLS, DELTA>(A, I, A.items[s], me, d);
 }
 
 // FP64 redo of the slots the mixed kernel queued (direct neighbor walk).
-template <int CLS, bool DELTA>
+template <int CLS, bool DELTA, int KT>
 __global__ void __launch_bounds__(256) k_lsa_fallback(LsaArgs A) {
   constexpr int NF = Layout<CLS, DELTA>::NF;
   const int count = *A.fb_count;
@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(256) k_lsa_fallback(LsaArgs A) {
     const Rec me = A.mir[s];
     const int lc = local_cell(I, me.x, me.y);
     double d[NF][5];
-    derivs64<CLS, DELTA>(A, I, lc, s, me, nullptr, 1 << 30, 0, 0, 0, d);
+    derivs64<CLS, DELTA, KT>(A, I, lc, s, me, nullptr, 1 << 30, 0, 0, 0, d);
     atomicAdd(&A.stats64[kFallbacks], 1ull);
     finish<CLS, DELTA>(A, I, A.items[s], me, d);
   }
@@ -458,13 +458,13 @@ struct PstArgs {
   int n, kt, list_len;
 };
 
-template <bool T, bool MIXED>
+template <bool T, bool MIXED, int KT>
 __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   constexpr int NF = T ? 4 : 3;  // u, v, rho (, tem)
   using R = typename std::conditional<MIXED, float, double>::type;
   extern __shared__ int32_t lds[];
   const int tid = threadIdx.x, B = blockDim.x;
-  const int s = blockIdx.x * B + tid;
+  const int s = xcd_block() * B + tid;
   if (s >= A.n) return;
   const int32_t inf = A.sinfo[s];
   if (info_ptype(inf) != 0) return;
@@ -490,8 +490,8 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
                    if constexpr (!MIXED) {
                      const double r = sqrt(sq_dist(dx, dy));
                      const double q = r * I.inv_h;
-                     const double w = I.w_norm * kshape_w<double>(kt, q);
-                     const double dwdr = I.w_norm * kshape_dw<double>(kt, q) * I.dw_fac;
+                     const double w = I.w_norm * kshape_w<double, KT>(kt, q);
+                     const double dwdr = I.w_norm * kshape_dw<double, KT>(kt, q) * I.dw_fac;
                      const double vj = nb.vol;
                      const double inv_r = 1.0 / r;
                      const double dwx = dwdr * (-dx) * inv_r * vj;
@@ -506,10 +506,10 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
                    } else {
                      const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
                      const float q = sqrtf(xs * xs + ys * ys);
-                     const float shp = kshape_w<float>(kt, q);
+                     const float shp = kshape_w<float, KT>(kt, q);
                      const float vwn = float(nb.vol * I.w_norm);
                      // dW/dr * (-x/r) * V = V * w_norm * dw_fac * shape'(q) * (-xs / q)
-                     const float g = float(nb.vol * I.w_norm * I.dw_fac) * kshape_dw<float>(kt, q) / q;
+                     const float g = float(nb.vol * I.w_norm * I.dw_fac) * kshape_dw<float, KT>(kt, q) / q;
                      const float wr = float(I.w_norm * inv_w_ave) * shp;
                      const float f = 1.0f + 0.2f * ((wr * wr) * (wr * wr));
                      sx = sx + f * (g * -xs);
@@ -618,18 +618,27 @@ __global__ void __launch_bounds__(256) k_dfmax(const Rec *__restrict__ rec, cons
 // ---------------------------------------------------------------------------
 static inline int blocks(int n, int b) { return (n + b - 1) / b; }
 
-template <int CLS, bool MIXED, bool DELTA>
+template <int CLS, bool MIXED, bool DELTA, int KT>
 static void lsa_launch(hipStream_t s, const LsaArgs &a, int block, size_t lds) {
-  hipLaunchKernelGGL((k_lsa<CLS, MIXED, DELTA>), dim3(blocks(a.n, block)), dim3(block), lds, s, a);
-  if (MIXED) hipLaunchKernelGGL((k_lsa_fallback<CLS, DELTA>), dim3(64), dim3(128), 0, s, a);
+  hipLaunchKernelGGL((k_lsa<CLS, MIXED, DELTA, KT>), dim3(xcd_grid(a.n, block)), dim3(block), lds, s, a);
+  if (MIXED) hipLaunchKernelGGL((k_lsa_fallback<CLS, DELTA, KT>), dim3(64), dim3(128), 0, s, a);
 }
 
+// Wendland C2 (the default kernel) is compiled in; other kernels take the
+// run-time switch.
 template <int CLS, bool DELTA>
-static void lsa_dispatch(hipStream_t s, const LsaArgs &a, int block, size_t lds, bool mixed) {
-  if (mixed)
-    lsa_launch<CLS, true, DELTA>(s, a, block, lds);
-  else
-    lsa_launch<CLS, false, DELTA>(s, a, block, lds);
+static void lsa_dispatch(hipStream_t s, const LsaArgs &a, int block, size_t lds, bool mixed, int kt) {
+  if (kt == 3) {
+    if (mixed)
+      lsa_launch<CLS, true, DELTA, 3>(s, a, block, lds);
+    else
+      lsa_launch<CLS, false, DELTA, 3>(s, a, block, lds);
+  } else {
+    if (mixed)
+      lsa_launch<CLS, true, DELTA, 0>(s, a, block, lds);
+    else
+      lsa_launch<CLS, false, DELTA, 0>(s, a, block, lds);
+  }
 }
 
 void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode) {
@@ -658,12 +667,13 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode) {
   if (mixed) launch_fill_i32(s, a.fb_count, 0, 1);
   const int cls = d.opt.target_problem == 1 ? 0 : (d.opt.has_tem ? 2 : 1);
   const bool delta = d.opt.delta != 0;
+  const int kt = d.opt.kernel_type;
   if (cls == 0)
-    lsa_dispatch<0, false>(s, a, d.block, lds, mixed);
+    lsa_dispatch<0, false>(s, a, d.block, lds, mixed, kt);
   else if (cls == 1)
-    delta ? lsa_dispatch<1, true>(s, a, d.block, lds, mixed) : lsa_dispatch<1, false>(s, a, d.block, lds, mixed);
+    delta ? lsa_dispatch<1, true>(s, a, d.block, lds, mixed, kt) : lsa_dispatch<1, false>(s, a, d.block, lds, mixed, kt);
   else
-    delta ? lsa_dispatch<2, true>(s, a, d.block, lds, mixed) : lsa_dispatch<2, false>(s, a, d.block, lds, mixed);
+    delta ? lsa_dispatch<2, true>(s, a, d.block, lds, mixed, kt) : lsa_dispatch<2, false>(s, a, d.block, lds, mixed, kt);
 }
 
 void launch_umax(hipStream_t s, Device &d, int cur) {
@@ -675,6 +685,15 @@ void launch_dfmax(hipStream_t s, Device &d, int cur) {
   launch_fill_u64(s, d.dfmax, 0ull, d.n_inst);
   hipLaunchKernelGGL(k_dfmax, dim3(d.red_blocks, d.n_inst), dim3(256), 0, s, d.rec[cur], d.info, d.f_prev,
                      d.inst, d.dfmax);
+}
+
+template <bool T, bool MIXED>
+static void pst_launch(hipStream_t s, const PstArgs &a, int block, size_t lds, int kt) {
+  const dim3 g(xcd_grid(a.n, block)), b(block);
+  if (kt == 3)
+    hipLaunchKernelGGL((k_pst<T, MIXED, 3>), g, b, lds, s, a);
+  else
+    hipLaunchKernelGGL((k_pst<T, MIXED, 0>), g, b, lds, s, a);
 }
 
 void launch_pst(hipStream_t s, Device &d, int cur) {
@@ -691,19 +710,12 @@ void launch_pst(hipStream_t s, Device &d, int cur) {
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;
   const size_t lds = sizeof(int32_t) * (size_t)d.list_len * d.block;
-  const dim3 g(blocks(d.n_total, d.block)), b(d.block);
   const bool mixed = d.opt.precision == LSSPH_MIXED;
-  if (d.opt.has_tem) {
-    if (mixed)
-      hipLaunchKernelGGL((k_pst<true, true>), g, b, lds, s, a);
-    else
-      hipLaunchKernelGGL((k_pst<true, false>), g, b, lds, s, a);
-  } else {
-    if (mixed)
-      hipLaunchKernelGGL((k_pst<false, true>), g, b, lds, s, a);
-    else
-      hipLaunchKernelGGL((k_pst<false, false>), g, b, lds, s, a);
-  }
+  const int kt = d.opt.kernel_type;
+  if (d.opt.has_tem)
+    mixed ? pst_launch<true, true>(s, a, d.block, lds, kt) : pst_launch<true, false>(s, a, d.block, lds, kt);
+  else
+    mixed ? pst_launch<false, true>(s, a, d.block, lds, kt) : pst_launch<false, false>(s, a, d.block, lds, kt);
 }
 
 }  // namespace lssph

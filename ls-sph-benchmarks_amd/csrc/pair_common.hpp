@@ -41,13 +41,15 @@ __device__ __forceinline__ double2 load_xy(const Rec *__restrict__ r, int j) {
 
 // Visits the slot ranges of the 3x3 stencil around local cell `lc` in the
 // reference's order (rows dy = -1..1, within a row dx = -1..1), passing the
-// periodic image code of each range (x code in bits 0-1, y code in 2-3).
+// periodic image code of each range (x code in bits 0-1, y code in 2-3).  A
+// row is one range, or two when a periodic row wraps; the loops are kept
+// rolled so the candidate loop body exists once in the code.
 template <typename F>
 __device__ __forceinline__ void visit_cells(const InstDev &I, int lc, const int32_t *__restrict__ cstart,
                                             F &&f) {
   const int nx = I.nx, ny = I.ny;
   const int cx = lc % nx, cy = lc / nx;
-#pragma unroll
+#pragma unroll 1
   for (int dy = -1; dy <= 1; ++dy) {
     int yy = cy + dy, ycode = 0;
     if (I.periodic) {
@@ -62,18 +64,31 @@ __device__ __forceinline__ void visit_cells(const InstDev &I, int lc, const int3
       continue;
     }
     const int row = I.cell_off + yy * nx;
-    const int yc = ycode << 2;
+    int a0, a1, b0 = 0, b1 = 0, ca = 0, cb = 0;
     if (!I.periodic) {
       const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < nx - 1 ? cx + 1 : nx - 1;
-      f(cstart[row + x0], cstart[row + x1 + 1], yc);
+      a0 = cstart[row + x0];
+      a1 = cstart[row + x1 + 1];
     } else if (cx == 0) {
-      f(cstart[row + nx - 1], cstart[row + nx], yc | 1);
-      f(cstart[row], cstart[row + 2], yc);
+      a0 = cstart[row + nx - 1];
+      a1 = cstart[row + nx];
+      ca = 1;
+      b0 = cstart[row];
+      b1 = cstart[row + 2];
     } else if (cx == nx - 1) {
-      f(cstart[row + cx - 1], cstart[row + cx + 1], yc);
-      f(cstart[row], cstart[row + 1], yc | 2);
+      a0 = cstart[row + cx - 1];
+      a1 = cstart[row + cx + 1];
+      b0 = cstart[row];
+      b1 = cstart[row + 1];
+      cb = 2;
     } else {
-      f(cstart[row + cx - 1], cstart[row + cx + 2], yc);
+      a0 = cstart[row + cx - 1];
+      a1 = cstart[row + cx + 2];
+    }
+#pragma unroll 1
+    for (int seg = 0; seg < 2; ++seg) {
+      const int t0 = seg ? b0 : a0, t1 = seg ? b1 : a1;
+      f(t0, t1, (ycode << 2) | (seg ? cb : ca));
     }
   }
 }
