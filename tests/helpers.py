@@ -71,6 +71,10 @@ class Golden:
         """The state the reference run started from (before the initial ghost fill)."""
         if self.has("inject_x"):
             return self.state("inject")
+        return self.setup_state()
+
+    def setup_state(self) -> particles.State:
+        """The reference's init_particle output, walls' u, v, pre (tem) zeroed."""
         st = self.state("setup")
         # walls: the reference's setup leaves wall u, v, pre (tem) unset until
         # the first ghost call; start them from zero
