@@ -72,7 +72,10 @@ class HipEvent:
 
 def run_gpu(param, st, precision: int, steps: int, warmup: int) -> dict:
     print(f"[bench] precision={'mixed' if precision else 'fp64'}: upload {st.num_int} particles", flush=True)
+    t_up = time.perf_counter()
     sim = simulation_from_state(param, st, precision=precision)
+    sim.synchronize()
+    up_ms = (time.perf_counter() - t_up) * 1e3
     sim.eager_step(warmup)
     sim.synchronize()
     sim.kernel_times()
@@ -88,12 +91,14 @@ def run_gpu(param, st, precision: int, steps: int, warmup: int) -> dict:
     sim.profile(False)
     kt = sim.kernel_times()
     stats = sim.stats()
+    t_down = time.perf_counter()
     out = sim.download()
+    down_ms = (time.perf_counter() - t_down) * 1e3
     sim.close()
     t = (warmup + steps) * param.dt
     l2 = workloads.tg_l2_error(param, out, t, st.num_int)
     print(f"[bench]   {steps} steps in {ms:.1f} ms (host wall {wall * 1e3:.1f} ms), L2={l2:.4e}", flush=True)
-    return {"ms": ms, "kt": kt, "stats": stats, "state": out, "l2": l2}
+    return {"ms": ms, "kt": kt, "stats": stats, "state": out, "l2": l2, "upload_ms": up_ms, "download_ms": down_ms}
 
 
 def cpu_baseline(threads: int) -> dict | None:
@@ -133,6 +138,21 @@ def cpu_baseline(threads: int) -> dict | None:
     except Exception as e:  # noqa: BLE001
         print(f"[bench] port baseline failed: {e}", flush=True)
         return None
+
+
+TRAFFIC_PROFILE = REPO / "profiles" / "kernels_tg2048_mixed_hbm_traffic.txt"
+
+
+def measured_traffic(kernel: str, n_part: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (counter readings scaled by the copy-kernel
+    calibration recorded in the same file), per particle x n_part."""
+    try:
+        text = TRAFFIC_PROFILE.read_text()
+    except OSError:
+        return None
+    m = re.search(rf"^{kernel} traffic_bytes_per_particle=(\S+)", text, re.M)
+    return None if m is None else float(m.group(1)) * n_part
 
 
 def copy_probe(nbytes: int) -> None:
@@ -177,8 +197,8 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the domain-decomposition design is described in DESIGN.md); run with --gpus 1.")
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(a domain decomposition is designed but not built); run with --gpus 1.")
         sys.exit(2)
 
     n = args.n
@@ -194,7 +214,8 @@ def main():
     achieved = bytes_per_launch / (lsa_avg_ms / 1e3) / 1e9
     valu_tflops = LSA_FLOPS_PER_PARTICLE * n_part / (lsa_avg_ms / 1e3) / 1e12
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("k_lsa", n_part),
+                "traffic_source": str(TRAFFIC_PROFILE.relative_to(REPO)),
                 "kernel": "k_lsa (fused LS type-A operator + RK4 stage update)",
                 "kernel_avg_ms": round(lsa_avg_ms, 4), "launches": int(kt.lsa_launches),
                 "bytes_per_launch": bytes_per_launch,
@@ -216,6 +237,10 @@ def main():
         "stats": {"fp64_fallbacks": int(mixed["stats"].fp64_fallbacks),
                   "list_overflows": int(mixed["stats"].list_overflows)},
         "roofline": roofline,
+        # host-buffer boundary: upload (pageable host -> HBM + first cell build) and download
+        "transfer": {"upload_ms": round(mixed["upload_ms"], 2), "download_ms": round(mixed["download_ms"], 2),
+                     "pcie_inclusive_value": round(n_part * args.steps / (
+                         (mixed["ms"] + mixed["upload_ms"] + mixed["download_ms"]) / 1e3), 1)},
     }
     if fp64 is not None:
         v64 = n_part * args.steps / (fp64["ms"] / 1e3)

@@ -67,8 +67,9 @@ typedef struct lssph_params {
   /* extensions (reference value in brackets) */
   double diff_coef;         /* factor on the Laplacian in the diffusion RHS [1] */
   double diff_source;       /* factor on the manufactured source term       [1] */
-  double cond_max;          /* mixed precision: condition estimate above which
-                               a particle is re-solved in full FP64 [1e5]     */
+  double cond_max;          /* mixed precision: pivot-ratio condition estimate
+                               above which a particle is re-solved in full
+                               FP64 (<= 0 selects 1e4)                        */
   /* model options (config.h macros) */
   int32_t target_problem;   /* TARGET_PROBLEM 1..4                            */
   int32_t kernel_type;      /* KERNEL_TYPE 1..5                               */

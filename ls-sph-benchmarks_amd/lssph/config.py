@@ -72,7 +72,7 @@ MACROS = {
     "DIFF_COEF": ("diff_coef", float, 1.0),
     "DIFF_SOURCE": ("diff_source", float, 1.0),
     "DT": ("dt_override", float, 0.0),
-    "COND_MAX": ("cond_max", float, 1.0e5),
+    "COND_MAX": ("cond_max", float, 1.0e4),
 }
 
 
@@ -143,7 +143,7 @@ class Config:
     diff_coef: float = 1.0
     diff_source: float = 1.0
     dt_override: float = 0.0
-    cond_max: float = 1.0e5
+    cond_max: float = 1.0e4
 
     @classmethod
     def from_macros(cls, macros: dict) -> "Config":
