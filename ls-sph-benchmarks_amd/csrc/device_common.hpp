@@ -57,6 +57,11 @@ struct Options {
   int32_t pad;
 };
 
+// Hardware square root and reciprocal of FP32 (1 ulp), for the mixed path's
+// FP32 pair terms, whose accuracy is set by the FP32 sums anyway.
+__device__ __forceinline__ float fast_sqrtf(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // ---------------------------------------------------------------------------
 // Kernel shapes (kernel_functions_mod.f90:26-146) as functions of q = r/h
 // without the normalisation factor.

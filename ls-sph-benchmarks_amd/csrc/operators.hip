@@ -241,7 +241,7 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
   walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, list_len, B, tid,
                  [&](const Rec &nb, double dx, double dy) {
                    const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
-                   const float q = sqrtf(xs * xs + ys * ys);
+                   const float q = fast_sqrtf(xs * xs + ys * ys);
                    const float vw = float(nb.vol * I.w_norm) * kshape_w<float, KT>(kt, q);
                    float a[5];
                    basis5s(xs, ys, a);
@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   const InstDev &I = A.inst[info_inst(inf)];
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
-  const int cnt = scan_candidates(A.mir, A.sinfo, A.cstart, I, lc, s, me.x, me.y, lds, A.list_len, B, tid, false);
+  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, s, me.x, me.y, lds, A.list_len, B, tid);
   if (cnt > A.list_len) {
     atomicAdd(&A.stats64[kOverflows], 1ull);
     atomicMax(&A.counters[kMaxNbr], cnt);
@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
   const int kt = A.kt;
-  const int cnt = scan_candidates(A.mir, A.sinfo, A.cstart, I, lc, s, me.x, me.y, lds, A.list_len, B, tid, false);
+  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, s, me.x, me.y, lds, A.list_len, B, tid);
   if (cnt > A.list_len) atomicAdd(&A.stats64[kOverflows], 1ull);
 
   LsSums<R, NF> S;
@@ -505,11 +505,11 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
                      S.add(vj * w, a, df);
                    } else {
                      const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
-                     const float q = sqrtf(xs * xs + ys * ys);
+                     const float q = fast_sqrtf(xs * xs + ys * ys);
                      const float shp = kshape_w<float, KT>(kt, q);
                      const float vwn = float(nb.vol * I.w_norm);
                      // dW/dr * (-x/r) * V = V * w_norm * dw_fac * shape'(q) * (-xs / q)
-                     const float g = float(nb.vol * I.w_norm * I.dw_fac) * kshape_dw<float, KT>(kt, q) / q;
+                     const float g = float(nb.vol * I.w_norm * I.dw_fac) * kshape_dw<float, KT>(kt, q) * fast_rcpf(q);
                      const float wr = float(I.w_norm * inv_w_ave) * shp;
                      const float f = 1.0f + 0.2f * ((wr * wr) * (wr * wr));
                      sx = sx + f * (g * -xs);

@@ -93,27 +93,63 @@ __device__ __forceinline__ void visit_cells(const InstDev &I, int lc, const int3
   }
 }
 
+// True when the lane's 3x3 stencil wraps across a periodic boundary, i.e.
+// some candidate range carries an image shift.
+__device__ __forceinline__ bool stencil_wraps(const InstDev &I, int lc) {
+  if (!I.periodic) return false;
+  const int cx = lc % I.nx, cy = lc / I.nx;
+  return cx == 0 || cx == I.nx - 1 || cy == 0 || cy == I.ny - 1;
+}
+
+// True when any active lane of the wavefront has a wrapping stencil.  Waves
+// with none take loops without the image-shift arithmetic; x_j + 0 == x_j,
+// so both loops form bit-identical offsets.
+__device__ __forceinline__ bool wave_wraps(const InstDev &I, int lc) {
+  return __ballot(stencil_wraps(I, lc)) != 0ull;
+}
+
 // Candidate scan of slot s: fills the LDS list (slot | code << kSlotBits),
-// returns the neighbor count.  fluid_only skips non-fluid candidates (the
-// ghost markers see only fluid, interpolate_LSSPH_B_mod.f90:95).
-__device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, const int32_t *__restrict__ sinfo,
-                                               const int32_t *__restrict__ cstart, const InstDev &I, int lc,
-                                               int s, double xi, double yi, int32_t *lds, int list_len, int B,
-                                               int tid, bool fluid_only) {
+// returns the neighbor count.  Two candidates per iteration, so two record
+// loads are in flight before the first test.
+__device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
+                                               const InstDev &I, int lc, int s, double xi, double yi,
+                                               int32_t *lds, int list_len, int B, int tid) {
   int cnt = 0;
-  visit_cells(I, lc, cstart, [&](int t0, int t1, int code) {
-    const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
-    for (int t = t0; t < t1; ++t) {
-      if (t == s) continue;
-      if (fluid_only && info_ptype(sinfo[t]) != 0) continue;
-      const double2 xy = load_xy(mir, t);
-      const double dx = (xy.x + sx) - xi, dy = (xy.y + sy) - yi;
-      if (sq_dist(dx, dy) <= I.h_eff_sq) {
-        if (cnt < list_len) lds[cnt * B + tid] = t | (code << kSlotBits);
-        ++cnt;
-      }
+  const double he2 = I.h_eff_sq;
+  auto accept = [&](int t, double dx, double dy, int code) {
+    if (sq_dist(dx, dy) <= he2 && t != s) {
+      if (cnt < list_len) lds[cnt * B + tid] = t | (code << kSlotBits);
+      ++cnt;
     }
-  });
+  };
+  if (!wave_wraps(I, lc)) {
+    visit_cells(I, lc, cstart, [&](int t0, int t1, int) {
+      int t = t0;
+      for (; t + 2 <= t1; t += 2) {
+        const double2 p0 = load_xy(mir, t), p1 = load_xy(mir, t + 1);
+        accept(t, p0.x - xi, p0.y - yi, 0);
+        accept(t + 1, p1.x - xi, p1.y - yi, 0);
+      }
+      if (t < t1) {
+        const double2 p0 = load_xy(mir, t);
+        accept(t, p0.x - xi, p0.y - yi, 0);
+      }
+    });
+  } else {
+    visit_cells(I, lc, cstart, [&](int t0, int t1, int code) {
+      const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
+      int t = t0;
+      for (; t + 2 <= t1; t += 2) {
+        const double2 p0 = load_xy(mir, t), p1 = load_xy(mir, t + 1);
+        accept(t, (p0.x + sx) - xi, (p0.y + sy) - yi, code);
+        accept(t + 1, (p1.x + sx) - xi, (p1.y + sy) - yi, code);
+      }
+      if (t < t1) {
+        const double2 p0 = load_xy(mir, t);
+        accept(t, (p0.x + sx) - xi, (p0.y + sy) - yi, code);
+      }
+    });
+  }
   return cnt;
 }
 
@@ -126,13 +162,20 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
                                                const int32_t *lds, int cnt, int list_len, int B, int tid,
                                                F &&f) {
   if (cnt <= list_len) {
-    for (int k = 0; k < cnt; ++k) {
-      const int e = lds[k * B + tid];
-      const int t = e & kSlotMask, code = e >> kSlotBits;
-      const Rec nb = mir[t];
-      const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
-      const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
-      f(nb, dx, dy);
+    if (!wave_wraps(I, lc)) {
+      for (int k = 0; k < cnt; ++k) {
+        const Rec nb = mir[lds[k * B + tid] & kSlotMask];
+        f(nb, nb.x - me.x, nb.y - me.y);
+      }
+    } else {
+      for (int k = 0; k < cnt; ++k) {
+        const int e = lds[k * B + tid];
+        const int t = e & kSlotMask, code = e >> kSlotBits;
+        const Rec nb = mir[t];
+        const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
+        const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
+        f(nb, dx, dy);
+      }
     }
   } else {
     visit_cells(I, lc, cstart, [&](int t0, int t1, int code) {
