@@ -22,8 +22,9 @@ struct lssph_handle {
   int cur = 0;
   bool uploaded = false, cells_ready = false;
   std::string err;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
-  int graph_end[2] = {0, 0};
+  // one captured step per starting state (record buffer cur, build buffers ib)
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  int graph_end[4] = {0, 0, 0, 0};
   int reorder_interval = 50;
   int64_t steps_since_reorder = 0;
   std::vector<void *> allocs;
@@ -167,6 +168,14 @@ int check_counters(lssph_handle *h) {
     std::snprintf(buf, sizeof buf, "%d particle(s) or marker(s) left the cell grid", c[kEscaped]);
     return fail(h, LSSPH_E_DOMAIN, buf);
   }
+  if (c[kStrayOverflow] > 0) {
+    char buf[200];
+    std::snprintf(buf, sizeof buf,
+                  "more than %d particles moved farther than one cell in one stage (%d over the limit); "
+                  "the time step is far above its stability limit",
+                  kStrayCap, c[kStrayOverflow]);
+    return fail(h, LSSPH_E_DOMAIN, buf);
+  }
   return 0;
 }
 
@@ -212,7 +221,7 @@ void enqueue_stage(lssph_handle *h, StageMode m) {
   // update_cell (skipped for the diffusion test, RK4_mod.f90:295-297, whose
   // particles do not move: only the mirror's values are refreshed)
   if (!is_diffusion(h))
-    timed(h, kGrpCells, [&] { launch_bin(h->stream, d, d.rec[h->cur], 0); });
+    timed(h, kGrpCells, [&] { launch_rebuild(h->stream, d, d.rec[h->cur]); });
   else
     timed(h, kGrpCells, [&] { launch_mirror(h->stream, d, d.rec[h->cur]); });
   if (d.n_ext) timed(h, kGrpGhost, [&] { launch_ghost(h->stream, d, h->cur); });
@@ -236,7 +245,7 @@ void enqueue_pst(lssph_handle *h) {
   timed(h, kGrpOther, [&] { launch_umax(h->stream, d, h->cur); });
   timed(h, kGrpPst, [&] { launch_pst(h->stream, d, h->cur); });
   h->cur ^= 1;
-  timed(h, kGrpCells, [&] { launch_bin(h->stream, d, d.rec[h->cur], 0); });
+  timed(h, kGrpCells, [&] { launch_rebuild(h->stream, d, d.rec[h->cur]); });
   if (d.n_ext) timed(h, kGrpGhost, [&] { launch_ghost(h->stream, d, h->cur); });
 }
 
@@ -270,7 +279,7 @@ int reorder(lssph_handle *h) {
 }
 
 void drop_graphs(lssph_handle *h) {
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
     if (h->graph[i]) {
       (void)hipGraphExecDestroy(h->graph[i]);
       h->graph[i] = nullptr;
@@ -353,8 +362,17 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   rc = rc ? rc : dalloc(h, &d.orig_of, n);
   rc = rc ? rc : dalloc(h, &d.phys_of, n);
   rc = rc ? rc : dalloc(h, &d.cell_count, d.n_cells);
-  rc = rc ? rc : dalloc(h, &d.cell_start, d.n_cells + 1);
-  rc = rc ? rc : dalloc(h, &d.items, n);
+  for (int b = 0; b < 2 && !rc; ++b) {
+    rc = dalloc(h, &d.cstart_buf[b], d.n_cells + 1);
+    rc = rc ? rc : dalloc(h, &d.items_buf[b], n);
+  }
+  rc = rc ? rc : dalloc(h, &d.cell_inst, d.n_cells);
+  rc = rc ? rc : dalloc(h, &d.kslot, n);
+  rc = rc ? rc : dalloc(h, &d.strays, 2 * kStrayCap);
+  rc = rc ? rc : dalloc(h, &d.nstray, 1);
+  rc = rc ? rc : dalloc(h, &d.sorig, n);
+  rc = rc ? rc : dalloc(h, &d.narr, d.n_cells);
+  rc = rc ? rc : dalloc(h, &d.arr, (size_t)kArrivals * d.n_cells);
   rc = rc ? rc : dalloc(h, &d.mirror, n);
   rc = rc ? rc : dalloc(h, &d.sinfo, n);
   rc = rc ? rc : dalloc(h, &d.slot_of, n);
@@ -393,13 +411,26 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
     hipError_t e = hipMemcpy(d.inst, h->inst_host.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = fail(h, LSSPH_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
   }
+  if (!rc) {
+    std::vector<int32_t> ci(d.n_cells);
+    for (int k = 0; k < n_inst; ++k) {
+      const InstDev &I = h->inst_host[k];
+      for (int c = 0; c < I.nx * I.ny; ++c) ci[I.cell_off + c] = k;
+    }
+    hipError_t e = hipMemcpy(d.cell_inst, ci.data(), sizeof(int32_t) * d.n_cells, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(h, LSSPH_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
   if (rc) {
     g_create_error = h->err;
     lssph_destroy(h);
     return rc;
   }
+  select_build(d, 0);
+  // both builds' slot arrays start with valid particle ids
+  for (int b = 0; b < 2; ++b) launch_fill_i32(h->stream, d.items_buf[b], 0, d.n_total);
   launch_fill_i32(h->stream, d.cell_count, 0, d.n_cells);
   launch_fill_i32(h->stream, d.counters, 0, kNumCounters);
+  launch_fill_i32(h->stream, d.nstray, 0, 1);
   launch_fill_u64(h->stream, d.stats64, 0ull, kNumStats64);
   *out = h;
   return 0;
@@ -570,15 +601,16 @@ int32_t lssph_step(lssph_handle_t h, int32_t nsteps) {
     if (h->reorder_interval > 0 && !is_diffusion(h) && h->steps_since_reorder >= h->reorder_interval) {
       if (int rc = reorder(h)) return rc;
     }
-    const int c0 = h->cur;
+    const int c0 = h->cur | (h->d.ib << 1);
     if (!h->graph[c0]) {
       hipGraph_t g = nullptr;
       HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       h->capturing = true;
       enqueue_step(h);
       h->capturing = false;
-      const int c1 = h->cur;
-      h->cur = c0;
+      const int c1 = h->cur | (h->d.ib << 1);
+      h->cur = c0 & 1;
+      select_build(h->d, c0 >> 1);
       HIP_TRY(h, hipStreamEndCapture(h->stream, &g));
       hipError_t e = hipGraphInstantiate(&h->graph[c0], g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -589,7 +621,8 @@ int32_t lssph_step(lssph_handle_t h, int32_t nsteps) {
       h->graph_end[c0] = c1;
     }
     HIP_TRY(h, hipGraphLaunch(h->graph[c0], h->stream));
-    h->cur = h->graph_end[c0];
+    h->cur = h->graph_end[c0] & 1;
+    select_build(h->d, h->graph_end[c0] >> 1);
     ++h->steps_since_reorder;
   }
   return 0;
@@ -686,7 +719,7 @@ int32_t lssph_stats(lssph_handle_t h, lssph_stats_t *out) {
 int32_t lssph_reset_stats(lssph_handle_t h) {
   if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
   launch_fill_u64(h->stream, h->d.stats64, 0ull, kNumStats64);
-  launch_fill_i32(h->stream, h->d.counters + kMaxCell, 0, kNumCounters - kMaxCell);
+  launch_fill_i32(h->stream, h->d.counters + kMaxCell, 0, kMaxNbr - kMaxCell + 1);
   HIP_TRY(h, hipGetLastError());
   return 0;
 }

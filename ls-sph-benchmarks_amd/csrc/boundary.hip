@@ -49,7 +49,7 @@ __device__ __forceinline__ void interp_lssph_b(const GhostArgs &A, const InstDev
   const double mx = A.vm_x[k], my = A.vm_y[k];
   const double inv_h = I.inv_h, inv_h2 = I.inv_h_sq;
   int num = 0;
-  visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int) {
+  visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int, int) {
     for (int t = t0; t < t1; ++t) {
       if (info_ptype(A.sinfo[t]) != 0) continue;
       const double2 xy = load_xy(A.mir, t);
@@ -104,7 +104,7 @@ __device__ __forceinline__ void interp_csph(const GhostArgs &A, const InstDev &I
   const double mx = A.vm_x[k], my = A.vm_y[k];
   double su = 0, sv = 0, sp = 0, st = 0, sw = 0;
   int num = 0;
-  visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int) {
+  visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int, int) {
     for (int t = t0; t < t1; ++t) {
       if (info_ptype(A.sinfo[t]) != 0) continue;
       const double2 xy = load_xy(A.mir, t);

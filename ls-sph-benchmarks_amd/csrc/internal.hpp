@@ -10,7 +10,13 @@
 namespace lssph {
 
 // Indices into Device::counters (int32) and Device::stats64 (uint64).
-enum { kEscaped = 0, kMaxCell = 1, kMaxNbr = 2, kFbCount = 3, kNumCounters = 4 };
+enum { kEscaped = 0, kMaxCell = 1, kMaxNbr = 2, kFbCount = 3, kStrayOverflow = 4, kNumCounters = 5 };
+
+// Incremental cell rebuild: particles whose new cell is outside the 3x3
+// stencil of their old cell are listed (old slot, new key) for the rebuild;
+// more than kStrayCap of them in one stage is reported as an error.
+constexpr int kStrayCap = 4096;
+constexpr int kArrivals = 8;  // arrivals per new cell merged directly
 enum { kFallbacks = 0, kOverflows = 1, kNumStats64 = 2 };
 
 // Raw device pointers and sizes; passed to the launchers.
@@ -24,8 +30,18 @@ struct Device {
   int32_t *orig_of = nullptr;       // [n_total] physical -> upload index
   int32_t *phys_of = nullptr;       // [n_total] upload index -> physical
   int32_t *cell_count = nullptr;    // [n_cells]
-  int32_t *cell_start = nullptr;    // [n_cells + 1]
-  int32_t *items = nullptr;         // [n_total] slot -> physical id (cells in order)
+  int32_t *cell_start = nullptr;    // [n_cells + 1] current build (points into cstart_buf)
+  int32_t *items = nullptr;         // [n_total] slot -> physical id, current build (items_buf)
+  int32_t *cstart_buf[2] = {};      // cell starts of the current and the next build
+  int32_t *items_buf[2] = {};
+  int ib = 0;                       // index of the current build's buffers
+  int32_t *cell_inst = nullptr;     // [n_cells] instance of each cell
+  int32_t *kslot = nullptr;         // [n_total] cell key per slot (movers write the new key)
+  int32_t *strays = nullptr;        // [2 * kStrayCap] (old slot, new key)
+  int32_t *nstray = nullptr;        // [1]
+  int32_t *sorig = nullptr;         // [n_total] upload index per slot
+  int32_t *narr = nullptr;          // [n_cells] arrivals per new cell
+  int32_t *arr = nullptr;           // [kArrivals * n_cells] their previous slots
   Rec *mirror = nullptr;            // [n_total] records in slot order
   int32_t *sinfo = nullptr;         // [n_total] info in slot order
   int32_t *slot_of = nullptr;       // [n_total] physical id -> slot
@@ -60,9 +76,12 @@ struct Device {
 void launch_fill_i32(hipStream_t s, int32_t *p, int32_t v, int n);
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n);
 int scan_tile_count(int n);
-void launch_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int32_t *tile_sums, int n);
+void launch_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int32_t *tile_sums, int n,
+                           int32_t *max_out);
 void launch_bin(hipStream_t s, Device &d, const Rec *rec, int fixed_key_mode);
+void launch_rebuild(hipStream_t s, Device &d, const Rec *rec);
 void launch_mirror(hipStream_t s, Device &d, const Rec *rec);
+void select_build(Device &d, int ib);
 void launch_gather_rec(hipStream_t s, int n, const int32_t *perm, const Rec *src, Rec *dst);
 void launch_gather_i32(hipStream_t s, int n, const int32_t *perm, const int32_t *src, int32_t *dst);
 void launch_gather_f64(hipStream_t s, int n, const int32_t *perm, const double *src, double *dst);

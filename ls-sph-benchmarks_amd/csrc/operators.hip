@@ -111,31 +111,42 @@ __device__ __forceinline__ void unpack(const LsSums<R, NF> &S, double (&A)[5][5]
     for (int i = 0; i < 5; ++i) B[f][i] = double(S.b[f][i]);
 }
 
-// FP64 LDL^T of the symmetric 5x5 M with one step of iterative refinement
-// (x += M^-1 (b - M x)).  Returns false when a pivot is not positive or the
-// pivot ratio max(d)/min(d) -- a cheap lower bound of cond_2(M) -- exceeds
-// cond_max; the caller then re-solves in full FP64.
+// Symmetric 5x5 entry (i, c) of packed upper-triangle sums.
+template <typename R>
+__device__ __forceinline__ double sym5(const R (&M)[15], int i, int c) {
+  return i >= c ? double(M[i * (i + 1) / 2 + c]) : double(M[c * (c + 1) / 2 + i]);
+}
+
+// FP64 LDL^T of the symmetric 5x5 moment matrix held as FP32 sums, with one
+// step of iterative refinement per right-hand side (x += M^-1 (b - M x),
+// the residual formed in FP64 against the FP32 sums).  The right-hand sides
+// are processed one at a time and the sums stay FP32 in registers, which
+// keeps the solve's register footprint near that of the pair loop.  Returns
+// false when a pivot is not positive or the pivot ratio max(d)/min(d) -- a
+// cheap lower bound of cond_2(M) -- exceeds cond_max; the caller then
+// re-solves in full FP64.
 template <int NF>
-__device__ __forceinline__ bool ldlt_solve_refine(const double (&A)[5][5], double (&X)[NF][5], double cond_max) {
-  double Lm[5][5], d[5], dinv[5];
-  bool ok = true;
+__device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, double (&X)[NF][5],
+                                                  double cond_max) {
+  double L[5][5], d[5], dinv[5];
   double dmax = 0.0, dmin = 1.0e300;
+  bool ok = true;
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
-    double s = A[j][j];
+    double dj = sym5(S.M, j, j);
 #pragma unroll
-    for (int k = 0; k < j; ++k) s -= Lm[j][k] * Lm[j][k] * d[k];
-    d[j] = s;
-    ok = ok && (s > 0.0);
-    dinv[j] = 1.0 / s;
-    dmax = fmax(dmax, s);
-    dmin = fmin(dmin, s);
+    for (int k = 0; k < j; ++k) dj -= L[j][k] * L[j][k] * d[k];
+    ok = ok && (dj > 0.0);
+    d[j] = dj;
+    dinv[j] = 1.0 / dj;
+    dmax = fmax(dmax, dj);
+    dmin = fmin(dmin, dj);
 #pragma unroll
     for (int i = j + 1; i < 5; ++i) {
-      double t = A[i][j];
+      double t = sym5(S.M, i, j);
 #pragma unroll
-      for (int k = 0; k < j; ++k) t -= Lm[i][k] * Lm[j][k] * d[k];
-      Lm[i][j] = t * dinv[j];
+      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k] * d[k];
+      L[i][j] = t * dinv[j];
     }
   }
   if (!ok || dmax > cond_max * dmin) return false;
@@ -143,25 +154,25 @@ __device__ __forceinline__ bool ldlt_solve_refine(const double (&A)[5][5], doubl
 #pragma unroll
     for (int i = 1; i < 5; ++i)
 #pragma unroll
-      for (int k = 0; k < i; ++k) y[i] -= Lm[i][k] * y[k];
+      for (int k = 0; k < i; ++k) y[i] -= L[i][k] * y[k];
 #pragma unroll
     for (int i = 0; i < 5; ++i) y[i] *= dinv[i];
 #pragma unroll
     for (int i = 3; i >= 0; --i)
 #pragma unroll
-      for (int k = i + 1; k < 5; ++k) y[i] -= Lm[k][i] * y[k];
+      for (int k = i + 1; k < 5; ++k) y[i] -= L[k][i] * y[k];
   };
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     double x[5], r[5];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) x[i] = X[f][i];
+    for (int i = 0; i < 5; ++i) x[i] = double(S.b[f][i]);
     solve(x);
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-      double t = X[f][i];
+      double t = double(S.b[f][i]);
 #pragma unroll
-      for (int c = 0; c < 5; ++c) t -= A[i][c] * x[c];
+      for (int c = 0; c < 5; ++c) t -= sym5(S.M, i, c) * x[c];
       r[i] = t;
     }
     solve(r);
@@ -188,6 +199,7 @@ struct LsaArgs {
   int32_t *counters;
   int32_t *fb_list;
   int32_t *fb_count;
+  MoveOut mv;
   int n, kt, list_len, mode;
 };
 
@@ -249,9 +261,8 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
                    field_diffs<CLS, DELTA>(nb, me, df);
                    S.add(vw, a, df);
                  });
-  double M[5][5], X[NF][5];
-  unpack(S, M, X);
-  if (!ldlt_solve_refine<NF>(M, X, I.cond_max)) return false;
+  double X[NF][5];
+  if (!ldlt_solve_refine<NF>(S, X, I.cond_max)) return false;
   scale_derivs<NF>(X, I, d);
   return true;
 }
@@ -259,7 +270,7 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
 // RHS of the governing equations (LSSPH_A_mod.f90:246-268) and the stage
 // update (RK2_mod.f90:77-107, RK4_mod.f90:216-286) of physical particle p.
 template <int CLS, bool DELTA>
-__device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p, const Rec &me,
+__device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p, int s, const Rec &me,
                                        const double (&d)[Layout<CLS, DELTA>::NF][5]) {
   using L = Layout<CLS, DELTA>;
   double r_u, r_v = 0.0, r_rho = 0.0, r_tem = 0.0;
@@ -388,6 +399,7 @@ __device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p
     out.vol = I.sp_mass / out.rho;
   }
   A.rout[p] = out;
+  if (!DIFF) record_move(A.mv, I, s, me.x, me.y, out.x, out.y);
 }
 
 template <int CLS, bool MIXED, bool DELTA, int KT>
@@ -423,7 +435,12 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
       derivs64<CLS, DELTA, KT>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d);
     }
   }
-  finish<CLS, DELTA>(A, I, A.items[s], me, d);
+  // Re-read the particle's own record for the epilogue instead of keeping it
+  // live across the solve (the compiler barrier forbids reusing the first
+  // load), which lowers the kernel's peak register count.
+  __asm__ volatile("" ::: "memory");
+  const Rec me2 = A.mir[s];
+  finish<CLS, DELTA>(A, I, A.items[s], s, me2, d);
 }
 
 // FP64 redo of the slots the mixed kernel queued (direct neighbor walk).
@@ -439,7 +456,7 @@ __global__ void __launch_bounds__(256) k_lsa_fallback(LsaArgs A) {
     double d[NF][5];
     derivs64<CLS, DELTA, KT>(A, I, lc, s, me, nullptr, 1 << 30, 0, 0, 0, d);
     atomicAdd(&A.stats64[kFallbacks], 1ull);
-    finish<CLS, DELTA>(A, I, A.items[s], me, d);
+    finish<CLS, DELTA>(A, I, A.items[s], s, me, d);
   }
 }
 
@@ -455,6 +472,7 @@ struct PstArgs {
   const InstDev *__restrict__ inst;
   const unsigned long long *__restrict__ umax;
   unsigned long long *stats64;
+  MoveOut mv;
   int n, kt, list_len;
 };
 
@@ -531,13 +549,14 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   // LS correction of u, v, rho (, tem) to the shifted position (:352-415)
   double X[NF][5];
   if (cnt >= 5) {
-    double M[5][5];
-    unpack(S, M, X);
-    if (!MIXED || !ldlt_solve_refine<NF>(M, X, I.cond_max)) {
-      if (MIXED) {
-        unpack(S, M, X);
-        atomicAdd(&A.stats64[kFallbacks], 1ull);
-      }
+    bool solved = false;
+    if constexpr (MIXED) {
+      solved = ldlt_solve_refine<NF>(S, X, I.cond_max);
+      if (!solved) atomicAdd(&A.stats64[kFallbacks], 1ull);
+    }
+    if (!solved) {
+      double M[5][5];
+      unpack(S, M, X);
       lu_solve<5, NF>(M, X);
     }
   } else {
@@ -567,6 +586,7 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   out.pre = I.c_sq * (out.rho - I.rho_ref);
   out.vol = I.sp_mass / out.rho;
   A.rout[A.items[s]] = out;
+  record_move(A.mv, I, s, me.x, me.y, out.x, out.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -616,7 +636,8 @@ __global__ void __launch_bounds__(256) k_dfmax(const Rec *__restrict__ rec, cons
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-static inline int blocks(int n, int b) { return (n + b - 1) / b; }
+// LDS of the neighbor lists: one 32-bit entry per neighbor.
+static size_t list_lds_bytes(int list_len, int block) { return (size_t)4 * list_len * block; }
 
 template <int CLS, bool MIXED, bool DELTA, int KT>
 static void lsa_launch(hipStream_t s, const LsaArgs &a, int block, size_t lds) {
@@ -658,11 +679,12 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode) {
   a.counters = d.counters;
   a.fb_list = d.fb_list;
   a.fb_count = d.counters + kFbCount;
+  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters};
   a.n = d.n_total;
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;
   a.mode = mode;
-  const size_t lds = sizeof(int32_t) * (size_t)d.list_len * d.block;
+  const size_t lds = list_lds_bytes(d.list_len, d.block);
   const bool mixed = d.opt.precision == LSSPH_MIXED;
   if (mixed) launch_fill_i32(s, a.fb_count, 0, 1);
   const int cls = d.opt.target_problem == 1 ? 0 : (d.opt.has_tem ? 2 : 1);
@@ -706,10 +728,11 @@ void launch_pst(hipStream_t s, Device &d, int cur) {
   a.inst = d.inst;
   a.umax = d.umax;
   a.stats64 = d.stats64;
+  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters};
   a.n = d.n_total;
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;
-  const size_t lds = sizeof(int32_t) * (size_t)d.list_len * d.block;
+  const size_t lds = list_lds_bytes(d.list_len, d.block);
   const bool mixed = d.opt.precision == LSSPH_MIXED;
   const int kt = d.opt.kernel_type;
   if (d.opt.has_tem)

@@ -41,9 +41,10 @@ __device__ __forceinline__ double2 load_xy(const Rec *__restrict__ r, int j) {
 
 // Visits the slot ranges of the 3x3 stencil around local cell `lc` in the
 // reference's order (rows dy = -1..1, within a row dx = -1..1), passing the
-// periodic image code of each range (x code in bits 0-1, y code in 2-3).  A
-// row is one range, or two when a periodic row wraps; the loops are kept
-// rolled so the candidate loop body exists once in the code.
+// periodic image code of each range (x code in bits 0-1, y code in 2-3) and
+// its range id 2 * (dy + 1) + segment.  A row is one range, or two when a
+// periodic row wraps; the loops are kept rolled so the candidate loop body
+// exists once in the code.
 template <typename F>
 __device__ __forceinline__ void visit_cells(const InstDev &I, int lc, const int32_t *__restrict__ cstart,
                                             F &&f) {
@@ -88,7 +89,7 @@ __device__ __forceinline__ void visit_cells(const InstDev &I, int lc, const int3
 #pragma unroll 1
     for (int seg = 0; seg < 2; ++seg) {
       const int t0 = seg ? b0 : a0, t1 = seg ? b1 : a1;
-      f(t0, t1, (ycode << 2) | (seg ? cb : ca));
+      f(t0, t1, (ycode << 2) | (seg ? cb : ca), 2 * (dy + 1) + seg);
     }
   }
 }
@@ -123,7 +124,7 @@ __device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, cons
     }
   };
   if (!wave_wraps(I, lc)) {
-    visit_cells(I, lc, cstart, [&](int t0, int t1, int) {
+    visit_cells(I, lc, cstart, [&](int t0, int t1, int, int) {
       int t = t0;
       for (; t + 2 <= t1; t += 2) {
         const double2 p0 = load_xy(mir, t), p1 = load_xy(mir, t + 1);
@@ -136,7 +137,7 @@ __device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, cons
       }
     });
   } else {
-    visit_cells(I, lc, cstart, [&](int t0, int t1, int code) {
+    visit_cells(I, lc, cstart, [&](int t0, int t1, int code, int) {
       const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
       int t = t0;
       for (; t + 2 <= t1; t += 2) {
@@ -178,7 +179,7 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
       }
     }
   } else {
-    visit_cells(I, lc, cstart, [&](int t0, int t1, int code) {
+    visit_cells(I, lc, cstart, [&](int t0, int t1, int code, int) {
       const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
       for (int t = t0; t < t1; ++t) {
         if (t == s) continue;
@@ -190,11 +191,10 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
   }
 }
 
-// Local cell of a fluid position: update_cell_mod.f90:53-55 (ceiling of
-// x * inv_h_eff, 1-based) in the closed box, floor(x * cells / L) in a
-// periodic one.  Returns -1 outside the grid.
-__device__ __forceinline__ int fluid_cell(const InstDev &I, double x, double y) {
-  int cx, cy;
+// Cell coordinates of a fluid position: update_cell_mod.f90:53-55 (ceiling
+// of x * inv_h_eff, 1-based) in the closed box, floor(x * cells / L) in a
+// periodic one.  Returns false outside the grid.
+__device__ __forceinline__ bool fluid_cell_xy(const InstDev &I, double x, double y, int &cx, int &cy) {
   if (I.periodic) {
     cx = (int)floor(x * I.inv_cw_x);
     cy = (int)floor(y * I.inv_cw_y);
@@ -204,7 +204,13 @@ __device__ __forceinline__ int fluid_cell(const InstDev &I, double x, double y) 
     cx = (int)ceil(x * I.inv_h_eff) - 1;
     cy = (int)ceil(y * I.inv_h_eff) - 1;
   }
-  if (cx < 0 || cx >= I.nx || cy < 0 || cy >= I.ny) return -1;
+  return cx >= 0 && cx < I.nx && cy >= 0 && cy < I.ny;
+}
+
+// Local cell index of a fluid position, -1 outside the grid.
+__device__ __forceinline__ int fluid_cell(const InstDev &I, double x, double y) {
+  int cx, cy;
+  if (!fluid_cell_xy(I, x, y, cx, cy)) return -1;
   return cx + cy * I.nx;
 }
 
@@ -221,6 +227,50 @@ __device__ __forceinline__ int fixed_cell(const InstDev &I, double x, double y) 
   const int cy = (int)ceil(y / I.h_eff) - 1;
   if (cx < 0 || cx >= I.nx || cy < 0 || cy >= I.ny) return -1;
   return cx + cy * I.nx;
+}
+
+// Where a particle that moved now belongs, for the incremental cell rebuild:
+// the new global cell key goes to kslot[s] (s = its slot in the build the
+// move started from); an escape is counted and clamped into the grid like
+// the full build does; a move that leaves the 3x3 stencil of the old cell
+// is queued on the stray list.
+struct MoveOut {
+  int32_t *kslot;
+  int32_t *strays;
+  int32_t *nstray;
+  int32_t *counters;
+};
+
+__device__ __forceinline__ void record_move(const MoveOut &M, const InstDev &I, int s, double ox, double oy,
+                                            double x, double y) {
+  int cx, cy, ocx, ocy;
+  if (!fluid_cell_xy(I, x, y, cx, cy)) {
+    atomicAdd(&M.counters[kEscaped], 1);
+    cx = cy = 0;
+  }
+  if (!fluid_cell_xy(I, ox, oy, ocx, ocy)) ocx = ocy = 0;  // as binned: clamped to cell 0
+  const int nx = I.nx, ny = I.ny;
+  int ddx = cx - ocx, ddy = cy - ocy;
+  if (I.periodic) {
+    ddx = ddx < 0 ? ddx + nx : ddx;
+    ddy = ddy < 0 ? ddy + ny : ddy;
+    ddx = ddx == nx - 1 ? 1 : ddx;
+    ddy = ddy == ny - 1 ? 1 : ddy;
+  } else {
+    ddx = ddx < 0 ? -ddx : ddx;
+    ddy = ddy < 0 ? -ddy : ddy;
+  }
+  const int key = I.cell_off + cx + cy * nx;
+  M.kslot[s] = key;
+  if (ddx > 1 || ddy > 1) {
+    const int k = atomicAdd(M.nstray, 1);
+    if (k < kStrayCap) {
+      M.strays[2 * k] = s;
+      M.strays[2 * k + 1] = key;
+    } else {
+      atomicAdd(&M.counters[kStrayOverflow], 1);
+    }
+  }
 }
 
 }  // namespace lssph

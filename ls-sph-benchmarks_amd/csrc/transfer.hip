@@ -16,7 +16,7 @@ __global__ void k_nbr_count(int n, const Rec *__restrict__ mir, const int32_t *_
   const InstDev &I = inst[info_inst(sinfo[s])];
   const double2 me = load_xy(mir, s);
   int cnt = 0;
-  visit_cells(I, local_cell(I, me.x, me.y), cstart, [&](int t0, int t1, int code) {
+  visit_cells(I, local_cell(I, me.x, me.y), cstart, [&](int t0, int t1, int code, int) {
     const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
     for (int t = t0; t < t1; ++t) {
       if (t == s) continue;
@@ -36,7 +36,7 @@ __global__ void k_nbr_fill(int n, const Rec *__restrict__ mir, const int32_t *__
   const InstDev &I = inst[info_inst(sinfo[s])];
   const double2 me = load_xy(mir, s);
   int64_t w = offsets[orig_of[items[s]]];
-  visit_cells(I, local_cell(I, me.x, me.y), cstart, [&](int t0, int t1, int code) {
+  visit_cells(I, local_cell(I, me.x, me.y), cstart, [&](int t0, int t1, int code, int) {
     const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
     for (int t = t0; t < t1; ++t) {
       if (t == s) continue;

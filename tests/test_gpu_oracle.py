@@ -124,3 +124,53 @@ def test_boussinesq_wall_model3_matches_oracle():
             s.step(1)
         e = _cmp(s.download(), o.get_state(), ("x", "y", "u", "v", "rho", "pre", "tem"))
         assert max(e.values()) <= TOL, e
+
+
+@pytest.mark.parametrize("case", ["periodic", "cavity"])
+def test_incremental_cell_builds_keep_reference_order(case):
+    """After steps (incremental builds after every stage and shift) the
+    neighbor lists, in summation order, equal the oracle's (which rebuilds
+    its cell lists from scratch) and those of a full rebuild."""
+    if case == "periodic":
+        param, st = workloads.taylor_green_periodic(n=48, re=100.0, rk=4, jitter=0.3, seed=21)
+    else:
+        param, st = workloads.cavity(n=24, re=100.0, rk=2)
+    o, s = _pair(param, st)
+    with s:
+        o.ghost()
+        s.ghost()
+        for _ in range(4):
+            o.step(1)
+            s.step(1)
+        co, lo = o.neighbors()
+        cs, ls = s.neighbors()
+        assert np.array_equal(co, cs) and np.array_equal(lo, ls)
+        np.testing.assert_array_equal(s.cell_index(), o.cell_index())
+        s.update_cell()
+        cf, lf = s.neighbors()
+        assert np.array_equal(cf, cs) and np.array_equal(lf, ls)
+        e = _cmp(s.download(), o.get_state(), ("x", "y", "u", "v", "rho", "pre"))
+        assert max(e.values()) <= TOL, e
+
+
+@pytest.mark.parametrize("rk", [2, 4])
+def test_far_moves_between_builds(rk):
+    """A uniform flow fast enough to carry particles one to three cells per
+    stage: the incremental build must take those particles from the stray
+    list (and select members one by one in cells with many arrivals); the
+    states and neighbor lists still equal the oracle's."""
+    param, st = workloads.taylor_green_periodic(n=24, re=100.0, rk=rk, jitter=0.3, seed=23)
+    cw = 1.0 / int(1.0 / param.h_eff)
+    speed = 3.0 * cw / param.dt
+    st.u[:] = speed
+    st.v[:] = 0.7 * speed
+    o, s = _pair(param, st)
+    with s:
+        for _ in range(2):
+            o.step(1)
+            s.step(1)
+            co, lo = o.neighbors()
+            cs, ls = s.neighbors()
+            assert np.array_equal(co, cs) and np.array_equal(lo, ls)
+        e = _cmp(s.download(), o.get_state(), ("x", "y", "u", "v", "rho", "pre"))
+        assert max(e.values()) <= TOL, e
