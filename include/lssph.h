@@ -196,6 +196,18 @@ int32_t lssph_synchronize(lssph_handle_t h);
 int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len,
                          int32_t reorder_interval);
 
+/* Cell-list skin as a fraction of h_eff (default 0.08).  The library has no
+   counterpart in the reference, whose update_cell (update_cell_mod.f90:
+   21-97) rebuilds h_eff-wide cells after every RK stage and shift: here the
+   cells are h_eff * (1 + skin) wide and one list serves the RK stages of a
+   time step (it is rebuilt after the shift, or after rk + 1 stages); every
+   pass still tests r^2 <= h_eff^2 at the current positions, so the
+   neighbor sets are unchanged.  A particle moving more than skin * h_eff / 2
+   between builds makes the next synchronising call fail with
+   LSSPH_E_DOMAIN.  0 rebuilds the cells after every pass, as the reference
+   does.  Call before lssph_upload. */
+int32_t lssph_set_skin(lssph_handle_t h, double skin);
+
 /* The HIP stream (hipStream_t) the handle launches on, for event timing. */
 void *lssph_stream(lssph_handle_t h);
 

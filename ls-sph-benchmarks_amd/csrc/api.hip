@@ -19,12 +19,16 @@ struct lssph_handle {
   std::vector<InstDev> inst_host;
   std::vector<int32_t> n_int, n_ext;
   std::vector<int32_t> inst_of;  // upload index -> instance
-  int cur = 0;
+  int cur = 0;          // records buffer holding the current state
+  int passes = 0;       // mover passes since the last cell build
+  int max_passes = 1;   // passes a cell list serves (rk + 1 with a skin, else 1)
+  double skin = 0.08;   // cell-grid skin as a fraction of h_eff
   bool uploaded = false, cells_ready = false;
   std::string err;
-  // one captured step per starting state (record buffer cur, build buffers ib)
-  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
-  int graph_end[4] = {0, 0, 0, 0};
+  // one captured step per starting state (records buffer, build buffers,
+  // passes since the build)
+  hipGraphExec_t graph[8] = {};
+  int graph_end[8] = {};
   int reorder_interval = 50;
   int64_t steps_since_reorder = 0;
   std::vector<void *> allocs;
@@ -55,6 +59,19 @@ int fail(lssph_handle *h, int code, const std::string &msg) {
     if (e_ != hipSuccess)                                                              \
       return fail(h, LSSPH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+template <typename T>
+void dfree(lssph_handle *h, T *&p) {
+  if (!p) return;
+  for (auto &a : h->allocs)
+    if (a == (void *)p) {
+      (void)hipFree(a);
+      a = h->allocs.back();
+      h->allocs.pop_back();
+      break;
+    }
+  p = nullptr;
+}
 
 template <typename T>
 int dalloc(lssph_handle *h, T **p, size_t count) {
@@ -104,7 +121,15 @@ int validate(const lssph_params_t *p, int n_inst, const int32_t *n_int, const in
   return 0;
 }
 
-InstDev make_inst(const lssph_params_t &p) {
+// Internal cell width: h_eff + skin * h_eff, reduced for periodic boxes so
+// that at least 3 cells fit per direction (skin >= 0).
+double cell_width(const lssph_params_t &p, double skin) {
+  double w = p.h_eff * (1.0 + skin);
+  if (p.periodic) w = std::min(w, std::min(p.len_x, p.len_y) / 3.0);
+  return std::max(w, p.h_eff);
+}
+
+InstDev make_inst(const lssph_params_t &p, double skin) {
   InstDev I{};
   I.h = p.h;
   I.inv_h = 1.0 / p.h;
@@ -146,15 +171,26 @@ InstDev make_inst(const lssph_params_t &p) {
   I.y_min = 0.0 + p.wl_thick;
   I.y_max = p.len_y + p.wl_thick;
   I.periodic = p.periodic;
+  const double w = cell_width(p, skin);
+  const double sk = w - p.h_eff;  // the skin this instance gets
+  I.skin_half_sq = 0.25 * sk * sk;
   if (p.periodic) {
-    I.nx = (int)std::floor(p.len_x / p.h_eff);
-    I.ny = (int)std::floor(p.len_y / p.h_eff);
+    I.nx = (int)std::floor(p.len_x / w);
+    I.ny = (int)std::floor(p.len_y / w);
     I.inv_cw_x = (double)I.nx / p.len_x;
     I.inv_cw_y = (double)I.ny / p.len_y;
+    I.rnx = (int)std::floor(p.len_x / p.h_eff);
+    I.rny = (int)std::floor(p.len_y / p.h_eff);
+    I.inv_rcw_x = (double)I.rnx / p.len_x;
+    I.inv_rcw_y = (double)I.rny / p.len_y;
   } else {
+    I.nx = (int)std::floor((p.len_x + 2.0 * p.wl_thick) / w) + 1;
+    I.ny = (int)std::floor((p.len_y + 2.0 * p.wl_thick) / w) + 1;
+    I.inv_cw_x = 1.0 / w;
+    I.inv_cw_y = 1.0 / w;
     // init_cell_mod.f90:36-41
-    I.nx = (int)std::ceil((p.len_x + 2.0 * p.wl_thick) / p.h_eff);
-    I.ny = (int)std::ceil((p.len_y + 2.0 * p.wl_thick) / p.h_eff);
+    I.rnx = (int)std::ceil((p.len_x + 2.0 * p.wl_thick) / p.h_eff);
+    I.rny = (int)std::ceil((p.len_y + 2.0 * p.wl_thick) / p.h_eff);
   }
   return I;
 }
@@ -167,6 +203,11 @@ int check_counters(lssph_handle *h) {
     char buf[160];
     std::snprintf(buf, sizeof buf, "%d particle(s) or marker(s) left the cell grid", c[kEscaped]);
     return fail(h, LSSPH_E_DOMAIN, buf);
+  }
+  if (c[kSkinExceeded] > 0) {
+    return fail(h, LSSPH_E_DOMAIN,
+                "a particle moved farther than half the cell-list skin between cell builds (lssph_set_skin "
+                "raises the skin; 0 rebuilds the cells after every pass)");
   }
   if (c[kStrayOverflow] > 0) {
     char buf[200];
@@ -212,18 +253,37 @@ void timed(lssph_handle *h, int group, F &&f) {
   h->ev_pending.push_back({a, b, group});
 }
 
-// One RK stage pass: fused operator + update, then update_cell and ghost on
-// the new state (RK2_mod.f90:67-126, RK4_mod.f90:205-306).
+// A mover pass (RK stage or shift) moves every fluid particle.  The
+// reference rebuilds its cell list after each (update_cell,
+// RK2_mod.f90:116, RK4_mod.f90:295, PST_mod.f90:101); here the cells are
+// h_eff + skin wide and one list serves max_passes passes -- a time step --
+// with every move checked against the skin, and every pass still tests
+// r^2 <= h_eff^2 at the current positions.  The pass that uses a list up
+// also writes the new cell keys, and the incremental build follows it.  The
+// shift (up to 0.2 h per particle) always ends a list.
+template <typename F>
+void mover_pass(lssph_handle *h, int group, bool ends_list, F &&launch) {
+  Device &d = h->d;
+  const bool moving = !is_diffusion(h);
+  const bool build = moving && (ends_list || h->passes + 1 >= h->max_passes);
+  const int guard = (moving && !build && h->max_passes > 1) ? 1 : 0;
+  timed(h, group, [&] { launch(guard, build ? 1 : 0); });
+  h->cur ^= 1;
+  if (!moving) return;
+  ++h->passes;
+  if (build) {
+    timed(h, kGrpCells, [&] { launch_rebuild(h->stream, d, h->cur, h->cur ^ 1); });
+    h->cur ^= 1;
+    h->passes = 0;
+  }
+}
+
+// One RK stage pass: fused operator + update, the cell list, and ghost on
+// the new state (RK2_mod.f90:67-126, RK4_mod.f90:205-306).  The diffusion
+// test's particles do not move (RK4_mod.f90:295-297).
 void enqueue_stage(lssph_handle *h, StageMode m) {
   Device &d = h->d;
-  timed(h, kGrpLsa, [&] { launch_lsa(h->stream, d, h->cur, m); });
-  h->cur ^= 1;
-  // update_cell (skipped for the diffusion test, RK4_mod.f90:295-297, whose
-  // particles do not move: only the mirror's values are refreshed)
-  if (!is_diffusion(h))
-    timed(h, kGrpCells, [&] { launch_rebuild(h->stream, d, d.rec[h->cur]); });
-  else
-    timed(h, kGrpCells, [&] { launch_mirror(h->stream, d, d.rec[h->cur]); });
+  mover_pass(h, kGrpLsa, false, [&](int guard, int keys) { launch_lsa(h->stream, d, h->cur, m, guard, keys); });
   if (d.n_ext) timed(h, kGrpGhost, [&] { launch_ghost(h->stream, d, h->cur); });
 }
 
@@ -243,9 +303,7 @@ void enqueue_rk(lssph_handle *h) {
 void enqueue_pst(lssph_handle *h) {
   Device &d = h->d;
   timed(h, kGrpOther, [&] { launch_umax(h->stream, d, h->cur); });
-  timed(h, kGrpPst, [&] { launch_pst(h->stream, d, h->cur); });
-  h->cur ^= 1;
-  timed(h, kGrpCells, [&] { launch_rebuild(h->stream, d, d.rec[h->cur]); });
+  mover_pass(h, kGrpPst, true, [&](int guard, int keys) { launch_pst(h->stream, d, h->cur, guard, keys); });
   if (d.n_ext) timed(h, kGrpGhost, [&] { launch_ghost(h->stream, d, h->cur); });
 }
 
@@ -254,15 +312,13 @@ void enqueue_step(lssph_handle *h) {
   if (!is_diffusion(h)) enqueue_pst(h);
 }
 
-// Physical re-sort: permutes every per-particle array into the current
-// (cell, upload index) order so that neighbor records stay contiguous.
+// Physical re-sort: permutes the per-particle arrays indexed by physical id
+// into the current slot order, so that the RK accumulators a slot touches
+// stay contiguous; slots then map to themselves.
 int reorder(lssph_handle *h) {
   Device &d = h->d;
   hipStream_t s = h->stream;
   const int n = d.n_total;
-  const int other = h->cur ^ 1;
-  launch_gather_rec(s, n, d.items, d.rec[h->cur], d.rec[other]);
-  HIP_TRY(h, hipMemcpyAsync(d.rec[h->cur], d.rec[other], sizeof(Rec) * n, hipMemcpyDeviceToDevice, s));
   launch_gather_i32(s, n, d.items, d.info, d.stage_i32);
   HIP_TRY(h, hipMemcpyAsync(d.info, d.stage_i32, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
   launch_gather_i32(s, n, d.items, d.key, d.stage_i32);
@@ -272,18 +328,76 @@ int reorder(lssph_handle *h) {
   launch_gather_f64(s, n, d.items, d.f_prev, d.stage_f64);
   HIP_TRY(h, hipMemcpyAsync(d.f_prev, d.stage_f64, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   launch_inverse_perm(s, n, d.orig_of, d.phys_of);
+  launch_iota(s, d.items, n);
+  launch_iota(s, d.slot_of, n);
   launch_vm_init(s, d, h->cur);
-  launch_bin(s, d, d.rec[h->cur], 0);
   h->steps_since_reorder = 0;
   return 0;
 }
 
+// Full cell build of the current records (new buffers become current).
+void full_build(lssph_handle *h, int fixed_key_mode) {
+  launch_bin(h->stream, h->d, h->cur, h->cur ^ 1, fixed_key_mode);
+  h->cur ^= 1;
+  h->passes = 0;
+}
+
 void drop_graphs(lssph_handle *h) {
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 8; ++i)
     if (h->graph[i]) {
       (void)hipGraphExecDestroy(h->graph[i]);
       h->graph[i] = nullptr;
     }
+}
+
+// Per-instance constants and the internal cell grid for the handle's skin;
+// returns the total cell count.  Passes per cell list: rk + 1 with a skin
+// (one build per time step), 1 without.
+int64_t setup_grid(lssph_handle *h) {
+  h->inst_host.clear();
+  int64_t nc = 0;
+  bool all_skin = true;
+  for (size_t k = 0; k < h->params.size(); ++k) {
+    InstDev I = make_inst(h->params[k], h->skin);
+    I.cell_off = (int32_t)nc;
+    nc += (int64_t)I.nx * I.ny;
+    all_skin = all_skin && I.skin_half_sq > 0.0;
+    h->inst_host.push_back(I);
+  }
+  h->d.n_cells = (int)std::min<int64_t>(nc, 1ll << 30);
+  h->max_passes = all_skin ? h->params[0].rk + 1 : 1;
+  return nc;
+}
+
+// Arrays sized by the cell grid; uploads the instance constants and the
+// cell -> instance map.
+int alloc_grid(lssph_handle *h) {
+  Device &d = h->d;
+  dfree(h, d.cell_count);
+  dfree(h, d.cstart_buf[0]);
+  dfree(h, d.cstart_buf[1]);
+  dfree(h, d.cell_inst);
+  dfree(h, d.narr);
+  dfree(h, d.arr);
+  dfree(h, d.scan_tmp);
+  int rc = 0;
+  rc = rc ? rc : dalloc(h, &d.cell_count, d.n_cells);
+  for (int b = 0; b < 2 && !rc; ++b) rc = dalloc(h, &d.cstart_buf[b], d.n_cells + 1);
+  rc = rc ? rc : dalloc(h, &d.cell_inst, d.n_cells);
+  rc = rc ? rc : dalloc(h, &d.narr, d.n_cells);
+  rc = rc ? rc : dalloc(h, &d.arr, (size_t)kArrivals * d.n_cells);
+  rc = rc ? rc : dalloc(h, &d.scan_tmp, scan_tile_count(d.n_cells) + 1);
+  if (rc) return rc;
+  HIP_TRY(h, hipMemcpy(d.inst, h->inst_host.data(), sizeof(InstDev) * d.n_inst, hipMemcpyHostToDevice));
+  std::vector<int32_t> ci(d.n_cells);
+  for (int k = 0; k < d.n_inst; ++k) {
+    const InstDev &I = h->inst_host[k];
+    for (int c = 0; c < I.nx * I.ny; ++c) ci[I.cell_off + c] = k;
+  }
+  HIP_TRY(h, hipMemcpy(d.cell_inst, ci.data(), sizeof(int32_t) * d.n_cells, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemset(d.cell_count, 0, sizeof(int32_t) * d.n_cells));
+  select_build(d, d.ib);
+  return 0;
 }
 
 int ready(lssph_handle *h) {
@@ -310,20 +424,12 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   h->n_ext.assign(n_ext, n_ext + n_inst);
   Device &d = h->d;
   d.n_inst = n_inst;
-  int64_t ni = 0, ne = 0, nc = 0, pb = 0;
+  int64_t ni = 0, ne = 0;
   for (int k = 0; k < n_inst; ++k) {
-    InstDev I = make_inst(params[k]);
-    I.cell_off = (int32_t)nc;
-    nc += (int64_t)I.nx * I.ny;
-    // physical order is sorted by cell, and cells are instance-major, so an
-    // instance owns the contiguous physical range [pbeg, pend)
-    I.pbeg = (int32_t)pb;
-    pb += (int64_t)n_int[k] + n_ext[k];
-    I.pend = (int32_t)pb;
-    h->inst_host.push_back(I);
     ni += n_int[k];
     ne += n_ext[k];
   }
+  const int64_t nc = setup_grid(h);
   if (ni + ne > (1ll << 30) || nc > (1ll << 30) || n_inst >= (1 << 23)) {
     delete h;
     return fail(nullptr, LSSPH_E_ARG, "lssph_create: problem too large for 32-bit indices");
@@ -331,7 +437,6 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   d.n_int = (int)ni;
   d.n_ext = (int)ne;
   d.n_total = (int)(ni + ne);
-  d.n_cells = (int)nc;
   const lssph_params_t &a = params[0];
   d.opt = Options{a.target_problem, a.kernel_type, a.wall_model, a.rk, a.u_bc_top, a.u_bc_bottom,
                   a.u_bc_left, a.u_bc_right, a.tem_bc_top, a.tem_bc_bottom, a.tem_bc_left, a.tem_bc_right,
@@ -355,29 +460,21 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   const size_t n = d.n_total;
   int rc = 0;
   rc = rc ? rc : dalloc(h, &d.inst, n_inst);
-  rc = rc ? rc : dalloc(h, &d.rec[0], n);
-  rc = rc ? rc : dalloc(h, &d.rec[1], n);
+  rc = rc ? rc : dalloc(h, &d.mir[0], n);
+  rc = rc ? rc : dalloc(h, &d.mir[1], n);
+  rc = rc ? rc : dalloc(h, &d.xbuild, 2 * n);
   rc = rc ? rc : dalloc(h, &d.info, n);
   rc = rc ? rc : dalloc(h, &d.key, n);
   rc = rc ? rc : dalloc(h, &d.orig_of, n);
   rc = rc ? rc : dalloc(h, &d.phys_of, n);
-  rc = rc ? rc : dalloc(h, &d.cell_count, d.n_cells);
-  for (int b = 0; b < 2 && !rc; ++b) {
-    rc = dalloc(h, &d.cstart_buf[b], d.n_cells + 1);
-    rc = rc ? rc : dalloc(h, &d.items_buf[b], n);
-  }
-  rc = rc ? rc : dalloc(h, &d.cell_inst, d.n_cells);
+  for (int b = 0; b < 2 && !rc; ++b) rc = dalloc(h, &d.items_buf[b], n);
   rc = rc ? rc : dalloc(h, &d.kslot, n);
   rc = rc ? rc : dalloc(h, &d.strays, 2 * kStrayCap);
   rc = rc ? rc : dalloc(h, &d.nstray, 1);
   rc = rc ? rc : dalloc(h, &d.sorig, n);
-  rc = rc ? rc : dalloc(h, &d.narr, d.n_cells);
-  rc = rc ? rc : dalloc(h, &d.arr, (size_t)kArrivals * d.n_cells);
-  rc = rc ? rc : dalloc(h, &d.mirror, n);
   rc = rc ? rc : dalloc(h, &d.sinfo, n);
   rc = rc ? rc : dalloc(h, &d.slot_of, n);
   rc = rc ? rc : dalloc(h, &d.fb_list, n);
-  rc = rc ? rc : dalloc(h, &d.scan_tmp, scan_tile_count(d.n_cells) + 1);
   rc = rc ? rc : dalloc(h, &d.perm_tmp, n);
   for (int f = 0; f < 6 && !rc; ++f) rc = dalloc(h, &d.old_[f], n);
   for (int f = 0; f < 6 && !rc; ++f) rc = dalloc(h, &d.new_[f], n);
@@ -407,19 +504,7 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) rc = fail(h, LSSPH_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
-  if (!rc) {
-    hipError_t e = hipMemcpy(d.inst, h->inst_host.data(), sizeof(InstDev) * n_inst, hipMemcpyHostToDevice);
-    if (e != hipSuccess) rc = fail(h, LSSPH_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-  }
-  if (!rc) {
-    std::vector<int32_t> ci(d.n_cells);
-    for (int k = 0; k < n_inst; ++k) {
-      const InstDev &I = h->inst_host[k];
-      for (int c = 0; c < I.nx * I.ny; ++c) ci[I.cell_off + c] = k;
-    }
-    hipError_t e = hipMemcpy(d.cell_inst, ci.data(), sizeof(int32_t) * d.n_cells, hipMemcpyHostToDevice);
-    if (e != hipSuccess) rc = fail(h, LSSPH_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-  }
+  rc = rc ? rc : alloc_grid(h);
   if (rc) {
     g_create_error = h->err;
     lssph_destroy(h);
@@ -428,7 +513,6 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   select_build(d, 0);
   // both builds' slot arrays start with valid particle ids
   for (int b = 0; b < 2; ++b) launch_fill_i32(h->stream, d.items_buf[b], 0, d.n_total);
-  launch_fill_i32(h->stream, d.cell_count, 0, d.n_cells);
   launch_fill_i32(h->stream, d.counters, 0, kNumCounters);
   launch_fill_i32(h->stream, d.nstray, 0, 1);
   launch_fill_u64(h->stream, d.stats64, 0ull, kNumStats64);
@@ -475,15 +559,15 @@ int32_t lssph_upload(lssph_handle_t h, const int32_t *ptype, const double *x, co
   HIP_TRY(h, hipMemcpyAsync(d.stage_i32, ptype, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
   HIP_TRY(h, hipMemcpyAsync(d.perm_tmp, h->inst_of.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
   launch_upload(s, d, d.stage_f64, d.stage_i32, d.perm_tmp, has_tem);
-  // identity permutation, then the initial (cell, index) sort
-  std::vector<int32_t> iota(n);
-  for (size_t i = 0; i < n; ++i) iota[i] = (int32_t)i;
-  HIP_TRY(h, hipMemcpyAsync(d.orig_of, iota.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-  HIP_TRY(h, hipStreamSynchronize(s));  // iota is a host temporary
+  // identity permutation (records in upload order in buffer 0), then the
+  // initial (cell, index) sort
+  launch_iota(s, d.orig_of, (int)n);
+  launch_iota(s, d.items, (int)n);
+  launch_iota(s, d.slot_of, (int)n);
   launch_fill_i32(s, d.counters, 0, kNumCounters);
-  launch_bin(s, d, d.rec[0], 1);
+  launch_fill_i32(s, d.cell_count, 0, d.n_cells);
   h->cur = 0;
-  HIP_TRY(h, hipMemcpyAsync(d.rec[1], d.rec[0], sizeof(Rec) * n, hipMemcpyDeviceToDevice, s));
+  full_build(h, 1);
   // f_prev = 1 (setup_closed_box_mod.f90:180-183); permuted along with the rest
   {
     std::vector<double> ones(n, 1.0);
@@ -491,6 +575,7 @@ int32_t lssph_upload(lssph_handle_t h, const int32_t *ptype, const double *x, co
     HIP_TRY(h, hipStreamSynchronize(s));
   }
   launch_inverse_perm(s, d.n_total, d.orig_of, d.phys_of);
+  HIP_TRY(h, hipStreamSynchronize(s));
   int rc = reorder(h);
   if (rc) return rc;
   h->uploaded = true;
@@ -531,7 +616,7 @@ int32_t lssph_download_cell_index(lssph_handle_t h, int32_t *idx_sp) {
 int32_t lssph_init_cell(lssph_handle_t h) {
   if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
   if (!h->uploaded) return fail(h, LSSPH_E_STATE, "call lssph_upload first");
-  launch_bin(h->stream, h->d, h->d.rec[h->cur], 0);
+  full_build(h, 0);
   h->cells_ready = true;
   HIP_TRY(h, hipGetLastError());
   return 0;
@@ -539,7 +624,7 @@ int32_t lssph_init_cell(lssph_handle_t h) {
 
 int32_t lssph_update_cell(lssph_handle_t h) {
   if (int rc = ready(h)) return rc;
-  launch_bin(h->stream, h->d, h->d.rec[h->cur], 0);
+  full_build(h, 0);
   HIP_TRY(h, hipGetLastError());
   return 0;
 }
@@ -553,7 +638,7 @@ int32_t lssph_ghost(lssph_handle_t h) {
 
 int32_t lssph_calculate_rhs(lssph_handle_t h) {
   if (int rc = ready(h)) return rc;
-  launch_lsa(h->stream, h->d, h->cur, kRhsOnly);
+  launch_lsa(h->stream, h->d, h->cur, kRhsOnly, 0, 0);
   HIP_TRY(h, hipGetLastError());
   return 0;
 }
@@ -601,6 +686,9 @@ int32_t lssph_step(lssph_handle_t h, int32_t nsteps) {
     if (h->reorder_interval > 0 && !is_diffusion(h) && h->steps_since_reorder >= h->reorder_interval) {
       if (int rc = reorder(h)) return rc;
     }
+    // the captured step depends on the records buffer, the build buffers and
+    // whether the cell list is fresh
+    if (h->passes != 0) full_build(h, 0);
     const int c0 = h->cur | (h->d.ib << 1);
     if (!h->graph[c0]) {
       hipGraph_t g = nullptr;
@@ -608,9 +696,10 @@ int32_t lssph_step(lssph_handle_t h, int32_t nsteps) {
       h->capturing = true;
       enqueue_step(h);
       h->capturing = false;
+      if (h->passes != 0) full_build(h, 0);  // a list that would outlive the step
       const int c1 = h->cur | (h->d.ib << 1);
       h->cur = c0 & 1;
-      select_build(h->d, c0 >> 1);
+      select_build(h->d, (c0 >> 1) & 1);
       HIP_TRY(h, hipStreamEndCapture(h->stream, &g));
       hipError_t e = hipGraphInstantiate(&h->graph[c0], g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -622,7 +711,8 @@ int32_t lssph_step(lssph_handle_t h, int32_t nsteps) {
     }
     HIP_TRY(h, hipGraphLaunch(h->graph[c0], h->stream));
     h->cur = h->graph_end[c0] & 1;
-    select_build(h->d, h->graph_end[c0] >> 1);
+    select_build(h->d, (h->graph_end[c0] >> 1) & 1);
+    h->passes = 0;
     ++h->steps_since_reorder;
   }
   return 0;
@@ -670,7 +760,7 @@ int32_t lssph_neighbors(lssph_handle_t h, int32_t *counts, int32_t *list, int64_
   Device &d = h->d;
   hipStream_t s = h->stream;
   int32_t *dcounts = nullptr;
-  int64_t *doff = nullptr;
+  int64_t *doff = nullptr, *dkey = nullptr;
   int32_t *dlist = nullptr;
   HIP_TRY(h, hipMalloc(&dcounts, sizeof(int32_t) * std::max(d.n_int, 1)));
   launch_neighbor_count(s, d, h->cur, dcounts);
@@ -685,11 +775,12 @@ int32_t lssph_neighbors(lssph_handle_t h, int32_t *counts, int32_t *list, int64_
     rc = fail(h, LSSPH_E_CAPACITY, "lssph_neighbors: list capacity too small");
   } else if (tot > 0) {
     if (hipMalloc(&doff, sizeof(int64_t) * (d.n_int + 1)) != hipSuccess ||
-        hipMalloc(&dlist, sizeof(int32_t) * tot) != hipSuccess) {
+        hipMalloc(&dlist, sizeof(int32_t) * tot) != hipSuccess ||
+        hipMalloc(&dkey, sizeof(int64_t) * tot) != hipSuccess) {
       rc = fail(h, LSSPH_E_NOMEM, "lssph_neighbors: hipMalloc failed");
     } else {
       (void)hipMemcpyAsync(doff, off.data(), sizeof(int64_t) * (d.n_int + 1), hipMemcpyHostToDevice, s);
-      launch_neighbor_fill(s, d, h->cur, doff, dlist);
+      launch_neighbor_fill(s, d, h->cur, doff, dlist, dkey);
       (void)hipMemcpyAsync(list, dlist, sizeof(int32_t) * tot, hipMemcpyDeviceToHost, s);
       if (hipStreamSynchronize(s) != hipSuccess) rc = fail(h, LSSPH_E_HIP, "lssph_neighbors: kernel failed");
     }
@@ -697,6 +788,7 @@ int32_t lssph_neighbors(lssph_handle_t h, int32_t *counts, int32_t *list, int64_
   (void)hipFree(dcounts);
   if (doff) (void)hipFree(doff);
   if (dlist) (void)hipFree(dlist);
+  if (dkey) (void)hipFree(dkey);
   return rc;
 }
 
@@ -746,6 +838,18 @@ int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len, int3
   }
   if (reorder_interval) h->reorder_interval = reorder_interval < 0 ? 0 : reorder_interval;
   drop_graphs(h);
+  return 0;
+}
+
+int32_t lssph_set_skin(lssph_handle_t h, double skin) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!(skin >= 0.0) || skin > 1.0) return fail(h, LSSPH_E_ARG, "lssph_set_skin: skin must be in [0, 1]");
+  if (h->uploaded) return fail(h, LSSPH_E_STATE, "lssph_set_skin: call before lssph_upload");
+  h->skin = skin;
+  drop_graphs(h);
+  setup_grid(h);
+  if (int rc = alloc_grid(h)) return rc;
+  HIP_TRY(h, hipGetLastError());
   return 0;
 }
 

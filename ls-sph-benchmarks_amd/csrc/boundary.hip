@@ -4,15 +4,14 @@
 // wall): interpolate the fluid values at the marker from its fluid
 // neighbors in the cell-sorted mirror (LS type B with q = 3 or 6, or CSPH),
 // then extrapolate them to the paired wall particle per wall type and
-// boundary condition.  The wall values are written to the mirror slot (read
-// by the next pair kernel) and to the physical record.
+// boundary condition.  The wall values are written to the wall particle's
+// slot of the current records (read by the next pair kernel).
 #include "pair_common.hpp"
 
 namespace lssph {
 
 struct GhostArgs {
-  Rec *__restrict__ mir;   // fluid read, wall slots written
-  Rec *__restrict__ rec;   // physical records of the current buffer (walls written)
+  Rec *__restrict__ mir;   // current records: fluid read, wall slots written
   const int32_t *__restrict__ sinfo;
   const int32_t *__restrict__ slot_of;
   const int32_t *__restrict__ cstart;
@@ -151,7 +150,8 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs A) {
   if (has_t) A.vm_tem[k] = val[3];
 
   const int pw = A.vm_wall[k];
-  Rec w = A.rec[pw];
+  const int sw = A.slot_of[pw];
+  Rec w = A.mir[sw];
   const int t = A.vm_ptype[k];
   const double vu = val[0], vv = val[1], vp = val[2], vt = val[3];
   if (o.target_problem == 1) {
@@ -222,13 +222,13 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs A) {
         break;
     }
   }
-  A.rec[pw] = w;
-  A.mir[A.slot_of[pw]] = w;
+  A.mir[sw] = w;
 }
 
 // Virtual markers from the wall positions (init_virtual_markers_mod.f90:
 // 61-137) and their fixed cells (init_cell_mod.f90:166-176).
-__global__ void k_vm_init(int n_int, int n_ext, const Rec *__restrict__ rec, const int32_t *__restrict__ info,
+__global__ void k_vm_init(int n_int, int n_ext, const Rec *__restrict__ rec, const int32_t *__restrict__ slot_of,
+                          const int32_t *__restrict__ info,
                           const int32_t *__restrict__ phys_of, const InstDev *__restrict__ inst, double *vm_x,
                           double *vm_y, double *vm_nx, double *vm_ny, int32_t *vm_ptype, int32_t *vm_inst,
                           int32_t *vm_cell, int32_t *vm_wall, int32_t *escaped) {
@@ -238,7 +238,7 @@ __global__ void k_vm_init(int n_int, int n_ext, const Rec *__restrict__ rec, con
   const int32_t inf = info[pw];
   const int t = info_ptype(inf), ii = info_inst(inf);
   const InstDev &I = inst[ii];
-  const double wx = rec[pw].x, wy = rec[pw].y;
+  const double wx = rec[slot_of[pw]].x, wy = rec[slot_of[pw]].y;
   const double r2 = 1.0 / sqrt(2.0);
   double x = wx, y = wy, nx = 0.0, ny = 0.0;
   switch (t) {
@@ -271,8 +271,7 @@ static inline int blocks(int n, int b) { return (n + b - 1) / b; }
 void launch_ghost(hipStream_t s, Device &d, int cur) {
   if (d.n_ext == 0) return;
   GhostArgs a;
-  a.mir = d.mirror;
-  a.rec = d.rec[cur];
+  a.mir = d.mir[cur];
   a.sinfo = d.sinfo;
   a.slot_of = d.slot_of;
   a.cstart = d.cell_start;
@@ -296,8 +295,8 @@ void launch_ghost(hipStream_t s, Device &d, int cur) {
 
 void launch_vm_init(hipStream_t s, Device &d, int cur) {
   if (d.n_ext == 0) return;
-  hipLaunchKernelGGL(k_vm_init, dim3(blocks(d.n_ext, 256)), dim3(256), 0, s, d.n_int, d.n_ext, d.rec[cur],
-                     d.info, d.phys_of, d.inst, d.vm_x, d.vm_y, d.vm_nx, d.vm_ny, d.vm_ptype, d.vm_inst,
+  hipLaunchKernelGGL(k_vm_init, dim3(blocks(d.n_ext, 256)), dim3(256), 0, s, d.n_int, d.n_ext, d.mir[cur],
+                     d.slot_of, d.info, d.phys_of, d.inst, d.vm_x, d.vm_y, d.vm_nx, d.vm_ny, d.vm_ptype, d.vm_inst,
                      d.vm_cell, d.vm_wall, d.counters + kEscaped);
 }
 

@@ -24,6 +24,11 @@ __global__ void k_fill_i32(int32_t *__restrict__ p, int32_t v, int n) {
   if (i < n) p[i] = v;
 }
 
+__global__ void k_iota(int32_t *__restrict__ p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
 __global__ void k_fill_u64(unsigned long long *__restrict__ p, unsigned long long v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -53,22 +58,25 @@ __device__ __forceinline__ Run lane_runs(int key) {
   return r;
 }
 
-// Keys of every particle and per-cell counts.  Fluid keys follow the
-// particle positions; wall keys are fixed at init (fixed_key_mode = 1
+// Keys of every particle and per-cell counts, over the slots of the current
+// order (positions from the records in that order).  Fluid keys follow the
+// particle positions; wall keys are fixed at upload (fixed_key_mode = 1
 // computes them).  Escaped particles are counted and clamped into the grid
 // so that memory stays in bounds.
 __global__ void __launch_bounds__(256) k_bin_count(int n, const Rec *__restrict__ rec,
+                                                   const int32_t *__restrict__ items,
                                                    const int32_t *__restrict__ info,
                                                    const InstDev *__restrict__ inst, int32_t *__restrict__ key,
                                                    int32_t *__restrict__ count, int32_t *__restrict__ escaped,
                                                    int fixed_key_mode) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   int k = -1 - (threadIdx.x & 63);  // distinct sentinel per idle lane
-  if (p < n) {
+  if (t < n) {
+    const int p = items[t];
     const int32_t inf = info[p];
     const InstDev &I = inst[info_inst(inf)];
     if (info_ptype(inf) == 0 || fixed_key_mode) {
-      const double2 xy = *reinterpret_cast<const double2 *>(&rec[p].x);
+      const double2 xy = *reinterpret_cast<const double2 *>(&rec[t].x);
       int c = info_ptype(inf) == 0 ? fluid_cell(I, xy.x, xy.y) : fixed_cell(I, xy.x, xy.y);
       if (c < 0) {
         atomicAdd(escaped, 1);
@@ -382,32 +390,31 @@ k_cell_fill(int ncells, const int32_t *__restrict__ cell_inst, const InstDev *__
     }
 }
 
-// Cell-sorted mirror of the records: mirror[t] = rec[items[t]], the info and
-// cell key of each slot, and the slot of every physical particle.  Also
-// empties the stray list for the next moves.
-__global__ void k_mirror(int n, const int32_t *__restrict__ items, const Rec *__restrict__ rec,
+// Records in the new slot order: dst[t] = src[previous slot of the particle
+// at t], with the info, cell key and upload index of each slot and the
+// position the cell list is built at.  slot_of[p] turns from the previous
+// slot into the new one (each thread reads and writes only its own p).
+// Also empties the stray list for the next moves.
+__global__ void k_mirror(int n, const int32_t *__restrict__ items, int32_t *__restrict__ slot_of,
                          const int32_t *__restrict__ info, const int32_t *__restrict__ key,
-                         const int32_t *__restrict__ orig_of, Rec *__restrict__ mirror,
+                         const int32_t *__restrict__ orig_of, const Rec *__restrict__ src, Rec *__restrict__ dst,
                          int32_t *__restrict__ sinfo, int32_t *__restrict__ kslot, int32_t *__restrict__ sorig,
-                         int32_t *__restrict__ slot_of, int32_t *__restrict__ nstray) {
+                         double *__restrict__ xbuild, int32_t *__restrict__ nstray) {
   const int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t == 0) *nstray = 0;
   if (t >= n) return;
   const int p = items[t];
-  mirror[t] = rec[p];
+  const Rec r = src[slot_of[p]];
+  dst[t] = r;
   sinfo[t] = info[p];
   kslot[t] = key[p];
   sorig[t] = orig_of[p];
   slot_of[p] = t;
+  xbuild[2 * t] = r.x;
+  xbuild[2 * t + 1] = r.y;
 }
 
 // --- permutations (physical re-sort) --------------------------------------
-__global__ void k_gather_rec(int n, const int32_t *__restrict__ perm, const Rec *__restrict__ src,
-                             Rec *__restrict__ dst) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) dst[t] = src[perm[t]];
-}
-
 __global__ void k_gather_i32(int n, const int32_t *__restrict__ perm, const int32_t *__restrict__ src,
                              int32_t *__restrict__ dst) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -432,6 +439,10 @@ void launch_fill_i32(hipStream_t s, int32_t *p, int32_t v, int n) {
   if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
+void launch_iota(hipStream_t s, int32_t *p, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_iota, dim3(blocks(n, 256)), dim3(256), 0, s, p, n);
+}
+
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n) {
   if (n > 0) hipLaunchKernelGGL(k_fill_u64, dim3(blocks(n, 256)), dim3(256), 0, s, p, v, n);
 }
@@ -446,21 +457,22 @@ void launch_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int32
   hipLaunchKernelGGL(k_scan_add, dim3(blocks(n, 256)), dim3(256), 0, s, out, tile_sums, n);
 }
 
-void launch_bin(hipStream_t s, Device &d, const Rec *rec, int fixed_key_mode) {
+static void launch_mirror(hipStream_t s, Device &d, int src, int dst) {
+  hipLaunchKernelGGL(k_mirror, dim3(xcd_grid(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.items, d.slot_of,
+                     d.info, d.key, d.orig_of, d.mir[src], d.mir[dst], d.sinfo, d.kslot, d.sorig, d.xbuild,
+                     d.nstray);
+}
+
+void launch_bin(hipStream_t s, Device &d, int src, int dst, int fixed_key_mode) {
   const int n = d.n_total;
-  hipLaunchKernelGGL(k_bin_count, dim3(blocks(n, 256)), dim3(256), 0, s, n, rec, d.info, d.inst, d.key,
-                     d.cell_count, d.counters + kEscaped, fixed_key_mode);
+  hipLaunchKernelGGL(k_bin_count, dim3(blocks(n, 256)), dim3(256), 0, s, n, d.mir[src], d.items, d.info, d.inst,
+                     d.key, d.cell_count, d.counters + kEscaped, fixed_key_mode);
   launch_exclusive_scan(s, d.cell_count, d.cell_start, d.scan_tmp, d.n_cells, d.counters + kMaxCell);
   hipLaunchKernelGGL(k_scatter, dim3(blocks(n, 256)), dim3(256), 0, s, n, d.key, d.cell_start,
                      d.cell_count, d.items);
   hipLaunchKernelGGL(k_cell_sort, dim3(blocks(d.n_cells, kSortThreads)), dim3(kSortThreads), 0, s, d.n_cells,
                      d.cell_start, d.items, d.orig_of);
-  launch_mirror(s, d, rec);
-}
-
-void launch_mirror(hipStream_t s, Device &d, const Rec *rec) {
-  hipLaunchKernelGGL(k_mirror, dim3(xcd_grid(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.items, rec, d.info,
-                     d.key, d.orig_of, d.mirror, d.sinfo, d.kslot, d.sorig, d.slot_of, d.nstray);
+  launch_mirror(s, d, src, dst);
 }
 
 void select_build(Device &d, int ib) {
@@ -469,9 +481,9 @@ void select_build(Device &d, int ib) {
   d.cell_start = d.cstart_buf[ib];
 }
 
-// Incremental build after a move (the movers have written kslot and the
-// stray list); the next build's buffers become current.
-void launch_rebuild(hipStream_t s, Device &d, const Rec *rec) {
+// Incremental build after moves (the last mover pass has written kslot and
+// the stray list); the next build's buffers become current.
+void launch_rebuild(hipStream_t s, Device &d, int src, int dst) {
   const int nxt = d.ib ^ 1;
   hipLaunchKernelGGL(k_cell_count, dim3(blocks(d.n_cells, 256)), dim3(256), 0, s, d.n_cells, d.cell_inst, d.inst,
                      d.cell_start, d.kslot, d.strays, d.nstray, d.cell_count, d.narr, d.arr);
@@ -480,11 +492,7 @@ void launch_rebuild(hipStream_t s, Device &d, const Rec *rec) {
                      d.cell_inst, d.inst, d.cell_start, d.cstart_buf[nxt], d.kslot, d.sorig, d.strays, d.nstray,
                      d.narr, d.arr, d.items, d.items_buf[nxt], d.key, d.cell_count);
   select_build(d, nxt);
-  launch_mirror(s, d, rec);
-}
-
-void launch_gather_rec(hipStream_t s, int n, const int32_t *perm, const Rec *src, Rec *dst) {
-  hipLaunchKernelGGL(k_gather_rec, dim3(blocks(n, 256)), dim3(256), 0, s, n, perm, src, dst);
+  launch_mirror(s, d, src, dst);
 }
 
 void launch_gather_i32(hipStream_t s, int n, const int32_t *perm, const int32_t *src, int32_t *dst) {

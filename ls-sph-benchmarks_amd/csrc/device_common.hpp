@@ -2,9 +2,10 @@
 // kernels for gfx950.
 //
 // Particle records live in HBM as 64-byte AoS records (one per particle) in
-// a cell-sorted physical order, double-buffered across RK stages: a pair
-// kernel reads every neighbor from buffer A and writes the updated particle
-// to buffer B, so the stage update is fused into the operator's epilogue.
+// cell order ("slots"), double-buffered across passes: a pair kernel reads
+// every neighbor from buffer A and writes the updated particle to the same
+// slot of buffer B, so the stage update is fused into the operator's
+// epilogue and the cell order survives until the next cell build.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,7 +34,10 @@ struct InstDev {
   double h, inv_h, inv_h2, h_eff_sq, inv_h_eff, h_eff;
   double inv_h_sq;    // inv_h * inv_h (type-B basis, interpolate_LSSPH_B_mod.f90:152)
   double threshold;   // steady-state threshold
-  double sp_mass, w_ave, wl_thick, len_x, len_y, inv_cw_x, inv_cw_y;
+  double sp_mass, w_ave, wl_thick, len_x, len_y;
+  double inv_cw_x, inv_cw_y;  // internal cell grid: 1 / cell width (>= h_eff + skin)
+  double inv_rcw_x, inv_rcw_y;  // periodic reference grid: cells / length
+  double skin_half_sq;        // (skin / 2)^2: largest squared move between cell builds
   double w_norm;      // kernel normalisation, e.g. 7/(4 pi h^2) for Wendland C2
   double dw_fac;      // dW/dr = w_norm * shape' * dw_fac
   double rho_ref, vis_ref, c_sq, thermal_dif, alpha_ref, gravity, tem_ave;
@@ -42,8 +46,8 @@ struct InstDev {
   double tem_top, tem_bottom, tem_left, tem_right;
   double diff_coef, diff_source, cond_max;
   double x_min, x_max, y_min, y_max;  // fluid box (walls beyond)
-  int32_t nx, ny, cell_off, periodic;
-  int32_t pbeg, pend;  // physical particle range of the instance
+  int32_t nx, ny, cell_off, periodic;  // internal cell grid
+  int32_t rnx, rny;                     // the reference's cell grid (cell_idx output)
   int32_t pad0, pad1;
 };
 

@@ -10,7 +10,15 @@
 namespace lssph {
 
 // Indices into Device::counters (int32) and Device::stats64 (uint64).
-enum { kEscaped = 0, kMaxCell = 1, kMaxNbr = 2, kFbCount = 3, kStrayOverflow = 4, kNumCounters = 5 };
+enum {
+  kEscaped = 0,
+  kMaxCell = 1,
+  kMaxNbr = 2,
+  kFbCount = 3,
+  kStrayOverflow = 4,
+  kSkinExceeded = 5,
+  kNumCounters = 6
+};
 
 // Incremental cell rebuild: particles whose new cell is outside the 3x3
 // stencil of their old cell are listed (old slot, new key) for the rebuild;
@@ -24,7 +32,7 @@ struct Device {
   int n_total = 0, n_int = 0, n_ext = 0, n_inst = 0, n_cells = 0;
   Options opt{};
   InstDev *inst = nullptr;          // [n_inst]
-  Rec *rec[2] = {nullptr, nullptr};  // double-buffered records, physical order
+  Rec *mir[2] = {nullptr, nullptr};  // records in slot order, double-buffered across passes
   int32_t *info = nullptr;          // [n_total] ptype | inst << 8
   int32_t *key = nullptr;           // [n_total] global cell id
   int32_t *orig_of = nullptr;       // [n_total] physical -> upload index
@@ -42,7 +50,7 @@ struct Device {
   int32_t *sorig = nullptr;         // [n_total] upload index per slot
   int32_t *narr = nullptr;          // [n_cells] arrivals per new cell
   int32_t *arr = nullptr;           // [kArrivals * n_cells] their previous slots
-  Rec *mirror = nullptr;            // [n_total] records in slot order
+  double *xbuild = nullptr;         // [2 * n_total] position per slot at the last cell build
   int32_t *sinfo = nullptr;         // [n_total] info in slot order
   int32_t *slot_of = nullptr;       // [n_total] physical id -> slot
   int32_t *fb_list = nullptr;       // [n_total] slots queued for the FP64 fallback
@@ -78,11 +86,12 @@ void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v,
 int scan_tile_count(int n);
 void launch_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int32_t *tile_sums, int n,
                            int32_t *max_out);
-void launch_bin(hipStream_t s, Device &d, const Rec *rec, int fixed_key_mode);
-void launch_rebuild(hipStream_t s, Device &d, const Rec *rec);
-void launch_mirror(hipStream_t s, Device &d, const Rec *rec);
+// Cell builds read the records of buffer src (in the previous build's slot
+// order) and write them, in the new order, to buffer dst.
+void launch_bin(hipStream_t s, Device &d, int src, int dst, int fixed_key_mode);
+void launch_rebuild(hipStream_t s, Device &d, int src, int dst);
 void select_build(Device &d, int ib);
-void launch_gather_rec(hipStream_t s, int n, const int32_t *perm, const Rec *src, Rec *dst);
+void launch_iota(hipStream_t s, int32_t *p, int n);
 void launch_gather_i32(hipStream_t s, int n, const int32_t *perm, const int32_t *src, int32_t *dst);
 void launch_gather_f64(hipStream_t s, int n, const int32_t *perm, const double *src, double *dst);
 void launch_inverse_perm(hipStream_t s, int n, const int32_t *orig_of, int32_t *phys_of);
@@ -90,14 +99,17 @@ void launch_inverse_perm(hipStream_t s, int n, const int32_t *orig_of, int32_t *
 // operators.hip
 // Stage codes of the fused LS type-A kernel.
 enum StageMode { kRhsOnly = 0, kRk2S1, kRk2S2, kRk4S1, kRk4S2, kRk4S3, kRk4S4 };
-void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode);
+// Mover passes read records buffer cur and write buffer cur ^ 1; guard
+// checks each move against the skin, keys prepares the next cell build.
+void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, int keys);
 void launch_umax(hipStream_t s, Device &d, int cur);
 void launch_dfmax(hipStream_t s, Device &d, int cur);
-void launch_pst(hipStream_t s, Device &d, int cur);
+void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys);
 void launch_ghost(hipStream_t s, Device &d, int cur);
 void launch_vm_init(hipStream_t s, Device &d, int cur);
 void launch_neighbor_count(hipStream_t s, Device &d, int cur, int32_t *counts_orig);
-void launch_neighbor_fill(hipStream_t s, Device &d, int cur, const int64_t *offsets_orig, int32_t *list);
+void launch_neighbor_fill(hipStream_t s, Device &d, int cur, const int64_t *offsets_orig, int32_t *list,
+                          int64_t *sort_key);
 void launch_steady(hipStream_t s, Device &d, int cur);
 void launch_download(hipStream_t s, Device &d, int cur);
 void launch_download_rhs(hipStream_t s, Device &d);

@@ -186,8 +186,8 @@ __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, do
 // LS type A + stage update
 // ---------------------------------------------------------------------------
 struct LsaArgs {
-  const Rec *__restrict__ mir;        // cell-sorted mirror (read)
-  Rec *__restrict__ rout;             // physical records (write)
+  const Rec *__restrict__ mir;        // records in slot order (read)
+  Rec *__restrict__ mout;             // records in slot order (write)
   const int32_t *__restrict__ sinfo;  // info in slot order
   const int32_t *__restrict__ items;  // slot -> physical id
   const int32_t *__restrict__ cstart;
@@ -398,8 +398,8 @@ __device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p
     out.pre = I.c_sq * (out.rho - I.rho_ref);  // equation of state
     out.vol = I.sp_mass / out.rho;
   }
-  A.rout[p] = out;
-  if (!DIFF) record_move(A.mv, I, s, me.x, me.y, out.x, out.y);
+  A.mout[s] = out;
+  if (!DIFF) record_move(A.mv, I, s, out.x, out.y);
 }
 
 template <int CLS, bool MIXED, bool DELTA, int KT>
@@ -410,7 +410,10 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   const int s = xcd_block() * B + tid;
   if (s >= A.n) return;
   const int32_t inf = A.sinfo[s];
-  if (info_ptype(inf) != 0) return;
+  if (info_ptype(inf) != 0) {
+    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];  // walls move on with the records
+    return;
+  }
   const InstDev &I = A.inst[info_inst(inf)];
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
@@ -465,7 +468,7 @@ __global__ void __launch_bounds__(256) k_lsa_fallback(LsaArgs A) {
 // ---------------------------------------------------------------------------
 struct PstArgs {
   const Rec *__restrict__ mir;
-  Rec *__restrict__ rout;
+  Rec *__restrict__ mout;
   const int32_t *__restrict__ sinfo;
   const int32_t *__restrict__ items;
   const int32_t *__restrict__ cstart;
@@ -485,7 +488,10 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   const int s = xcd_block() * B + tid;
   if (s >= A.n) return;
   const int32_t inf = A.sinfo[s];
-  if (info_ptype(inf) != 0) return;
+  if (info_ptype(inf) != 0) {
+    A.mout[s] = A.mir[s];  // walls move on with the records
+    return;
+  }
   const int ii = info_inst(inf);
   const InstDev &I = A.inst[ii];
   const Rec me = A.mir[s];
@@ -585,8 +591,8 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   if constexpr (T) out.tem = me.tem + acc[3];
   out.pre = I.c_sq * (out.rho - I.rho_ref);
   out.vol = I.sp_mass / out.rho;
-  A.rout[A.items[s]] = out;
-  record_move(A.mv, I, s, me.x, me.y, out.x, out.y);
+  A.mout[s] = out;
+  record_move(A.mv, I, s, out.x, out.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -607,13 +613,16 @@ __device__ __forceinline__ double block_max(double v) {
 }
 
 // U_max of PST_mod.f90:49-53.
-__global__ void __launch_bounds__(256) k_umax(const Rec *__restrict__ rec, const int32_t *__restrict__ info,
-                                              const InstDev *__restrict__ inst, unsigned long long *umax) {
+// An instance's particles occupy the contiguous slot range of its cells.
+__global__ void __launch_bounds__(256) k_umax(const Rec *__restrict__ rec, const int32_t *__restrict__ sinfo,
+                                              const int32_t *__restrict__ cstart, const InstDev *__restrict__ inst,
+                                              unsigned long long *umax) {
   const InstDev &I = inst[blockIdx.y];
+  const int t0 = cstart[I.cell_off], t1 = cstart[I.cell_off + I.nx * I.ny];
   double s = 0.0;
-  for (int p = I.pbeg + blockIdx.x * blockDim.x + threadIdx.x; p < I.pend; p += gridDim.x * blockDim.x) {
-    if (info_ptype(info[p]) == 0) {
-      const double2 uv = *reinterpret_cast<const double2 *>(&rec[p].u);
+  for (int t = t0 + blockIdx.x * blockDim.x + threadIdx.x; t < t1; t += gridDim.x * blockDim.x) {
+    if (info_ptype(sinfo[t]) == 0) {
+      const double2 uv = *reinterpret_cast<const double2 *>(&rec[t].u);
       s = fmax(s, sqrt(uv.x * uv.x + uv.y * uv.y));
     }
   }
@@ -621,14 +630,17 @@ __global__ void __launch_bounds__(256) k_umax(const Rec *__restrict__ rec, const
   if (threadIdx.x == 0) atomic_max_nonneg(&umax[blockIdx.y], m);
 }
 
-// max|f_previous - u| of check_steady_state_mod.f90:36-46.
-__global__ void __launch_bounds__(256) k_dfmax(const Rec *__restrict__ rec, const int32_t *__restrict__ info,
-                                               const double *__restrict__ f_prev,
+// max|f_previous - u| of check_steady_state_mod.f90:36-46 (f_previous in
+// physical order).
+__global__ void __launch_bounds__(256) k_dfmax(const Rec *__restrict__ rec, const int32_t *__restrict__ sinfo,
+                                               const int32_t *__restrict__ items,
+                                               const int32_t *__restrict__ cstart, const double *__restrict__ f_prev,
                                                const InstDev *__restrict__ inst, unsigned long long *dfmax) {
   const InstDev &I = inst[blockIdx.y];
+  const int t0 = cstart[I.cell_off], t1 = cstart[I.cell_off + I.nx * I.ny];
   double s = 0.0;
-  for (int p = I.pbeg + blockIdx.x * blockDim.x + threadIdx.x; p < I.pend; p += gridDim.x * blockDim.x)
-    if (info_ptype(info[p]) == 0) s = fmax(s, fabs(f_prev[p] - rec[p].u));
+  for (int t = t0 + blockIdx.x * blockDim.x + threadIdx.x; t < t1; t += gridDim.x * blockDim.x)
+    if (info_ptype(sinfo[t]) == 0) s = fmax(s, fabs(f_prev[items[t]] - rec[t].u));
   const double m = block_max(s);
   if (threadIdx.x == 0) atomic_max_nonneg(&dfmax[blockIdx.y], m);
 }
@@ -662,10 +674,10 @@ static void lsa_dispatch(hipStream_t s, const LsaArgs &a, int block, size_t lds,
   }
 }
 
-void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode) {
+void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, int keys) {
   LsaArgs a;
-  a.mir = d.mirror;
-  a.rout = d.rec[cur ^ 1];
+  a.mir = d.mir[cur];
+  a.mout = d.mir[cur ^ 1];
   a.sinfo = d.sinfo;
   a.items = d.items;
   a.cstart = d.cell_start;
@@ -679,7 +691,7 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode) {
   a.counters = d.counters;
   a.fb_list = d.fb_list;
   a.fb_count = d.counters + kFbCount;
-  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters};
+  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters, d.xbuild, guard, keys};
   a.n = d.n_total;
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;
@@ -700,13 +712,14 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode) {
 
 void launch_umax(hipStream_t s, Device &d, int cur) {
   launch_fill_u64(s, d.umax, 0ull, d.n_inst);
-  hipLaunchKernelGGL(k_umax, dim3(d.red_blocks, d.n_inst), dim3(256), 0, s, d.rec[cur], d.info, d.inst, d.umax);
+  hipLaunchKernelGGL(k_umax, dim3(d.red_blocks, d.n_inst), dim3(256), 0, s, d.mir[cur], d.sinfo, d.cell_start,
+                     d.inst, d.umax);
 }
 
 void launch_dfmax(hipStream_t s, Device &d, int cur) {
   launch_fill_u64(s, d.dfmax, 0ull, d.n_inst);
-  hipLaunchKernelGGL(k_dfmax, dim3(d.red_blocks, d.n_inst), dim3(256), 0, s, d.rec[cur], d.info, d.f_prev,
-                     d.inst, d.dfmax);
+  hipLaunchKernelGGL(k_dfmax, dim3(d.red_blocks, d.n_inst), dim3(256), 0, s, d.mir[cur], d.sinfo, d.items,
+                     d.cell_start, d.f_prev, d.inst, d.dfmax);
 }
 
 template <bool T, bool MIXED>
@@ -718,17 +731,17 @@ static void pst_launch(hipStream_t s, const PstArgs &a, int block, size_t lds, i
     hipLaunchKernelGGL((k_pst<T, MIXED, 0>), g, b, lds, s, a);
 }
 
-void launch_pst(hipStream_t s, Device &d, int cur) {
+void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
   PstArgs a;
-  a.mir = d.mirror;
-  a.rout = d.rec[cur ^ 1];
+  a.mir = d.mir[cur];
+  a.mout = d.mir[cur ^ 1];
   a.sinfo = d.sinfo;
   a.items = d.items;
   a.cstart = d.cell_start;
   a.inst = d.inst;
   a.umax = d.umax;
   a.stats64 = d.stats64;
-  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters};
+  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters, d.xbuild, guard, keys};
   a.n = d.n_total;
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;

@@ -1,11 +1,13 @@
 // pair_common.hpp -- neighbor traversal shared by the pair kernels.
 //
-// Neighbor data comes from the cell-sorted mirror: after every cell build
-// the particle records are gathered into slot order (cells consecutive, each
-// cell's particles in the reference's list order, update_cell_mod.f90:
-// 57-92), so the 3x3 stencil of a particle is at most three contiguous slot
-// ranges (four to six when a periodic row wraps) and lanes of a wavefront
-// read the same or adjacent cache lines.
+// Neighbor data comes from the records in slot order (cells consecutive,
+// each cell's particles in ascending upload index like the reference's
+// lists, update_cell_mod.f90:57-92), so the 3x3 stencil of a particle is at
+// most three contiguous slot ranges (four to six when a periodic row wraps)
+// and lanes of a wavefront read the same or adjacent cache lines.  Cells are
+// h_eff + skin wide and rebuilt once per time step; every pass still tests
+// r_ij^2 <= h_eff^2 at the current positions, so the neighbor sets are the
+// reference's.
 #pragma once
 
 #include "internal.hpp"
@@ -191,23 +193,22 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
   }
 }
 
-// Cell coordinates of a fluid position: update_cell_mod.f90:53-55 (ceiling
-// of x * inv_h_eff, 1-based) in the closed box, floor(x * cells / L) in a
-// periodic one.  Returns false outside the grid.
+// Cell coordinates on the internal grid: cells of width h_eff + skin
+// (floor(x / width)), so that the 3x3 stencil of a cell list built at one
+// position still holds every pair within h_eff after each particle has moved
+// by up to skin / 2.  Periodic boxes clamp the upper edge.  Returns false
+// outside the grid.
 __device__ __forceinline__ bool fluid_cell_xy(const InstDev &I, double x, double y, int &cx, int &cy) {
+  cx = (int)floor(x * I.inv_cw_x);
+  cy = (int)floor(y * I.inv_cw_y);
   if (I.periodic) {
-    cx = (int)floor(x * I.inv_cw_x);
-    cy = (int)floor(y * I.inv_cw_y);
     cx = (cx == I.nx) ? I.nx - 1 : cx;
     cy = (cy == I.ny) ? I.ny - 1 : cy;
-  } else {
-    cx = (int)ceil(x * I.inv_h_eff) - 1;
-    cy = (int)ceil(y * I.inv_h_eff) - 1;
   }
   return cx >= 0 && cx < I.nx && cy >= 0 && cy < I.ny;
 }
 
-// Local cell index of a fluid position, -1 outside the grid.
+// Local cell index on the internal grid, -1 outside.
 __device__ __forceinline__ int fluid_cell(const InstDev &I, double x, double y) {
   int cx, cy;
   if (!fluid_cell_xy(I, x, y, cx, cy)) return -1;
@@ -220,37 +221,68 @@ __device__ __forceinline__ int local_cell(const InstDev &I, double x, double y) 
   return c < 0 ? 0 : c;
 }
 
-// Cell of a wall particle or virtual marker: init_cell_mod.f90:76-78 and
-// :170-172 divide by h_eff instead of multiplying by its inverse.
-__device__ __forceinline__ int fixed_cell(const InstDev &I, double x, double y) {
-  const int cx = (int)ceil(x / I.h_eff) - 1;
-  const int cy = (int)ceil(y / I.h_eff) - 1;
-  if (cx < 0 || cx >= I.nx || cy < 0 || cy >= I.ny) return -1;
-  return cx + cy * I.nx;
+// Walls and virtual markers use the same internal grid.
+__device__ __forceinline__ int fixed_cell(const InstDev &I, double x, double y) { return fluid_cell(I, x, y); }
+
+// The reference's 1-based cell index (cell%idx_SP): fluid particles
+// ceiling(x * inv_h_eff) (update_cell_mod.f90:53-55), walls and markers
+// ceiling(x / h_eff) (init_cell_mod.f90:76-78); periodic boxes (extension)
+// floor(x * cells / L).
+__device__ __forceinline__ int ref_cell_index(const InstDev &I, double x, double y, bool wall) {
+  int cx, cy;
+  if (I.periodic) {
+    cx = (int)floor(x * I.inv_rcw_x);
+    cy = (int)floor(y * I.inv_rcw_y);
+    cx = (cx == I.rnx) ? I.rnx - 1 : cx;
+    cy = (cy == I.rny) ? I.rny - 1 : cy;
+    return cx + 1 + cy * I.rnx;
+  }
+  if (wall) {
+    cx = (int)ceil(x / I.h_eff);
+    cy = (int)ceil(y / I.h_eff);
+  } else {
+    cx = (int)ceil(x * I.inv_h_eff);
+    cy = (int)ceil(y * I.inv_h_eff);
+  }
+  return cx + (cy - 1) * I.rnx;
 }
 
-// Where a particle that moved now belongs, for the incremental cell rebuild:
-// the new global cell key goes to kslot[s] (s = its slot in the build the
-// move started from); an escape is counted and clamped into the grid like
-// the full build does; a move that leaves the 3x3 stencil of the old cell
-// is queued on the stray list.
+// Bookkeeping of a moved particle at slot s between cell builds.
+//   guard:  the particle may move at most skin / 2 from where the current
+//           cell list was built (xbuild); beyond that the 3x3 stencil could
+//           miss pairs, so the move is counted as an error;
+//   keys:   in the pass right before a build, the new cell key goes to
+//           kslot[s] (which held the key at the build); an escape is counted
+//           and clamped into the grid like the full build does; a move that
+//           leaves the 3x3 stencil of the old cell is queued on the stray
+//           list.
 struct MoveOut {
   int32_t *kslot;
   int32_t *strays;
   int32_t *nstray;
   int32_t *counters;
+  const double *xbuild;  // [2 * n] position at the last cell build, per slot
+  int guard, keys;
 };
 
-__device__ __forceinline__ void record_move(const MoveOut &M, const InstDev &I, int s, double ox, double oy,
-                                            double x, double y) {
-  int cx, cy, ocx, ocy;
+__device__ __forceinline__ void record_move(const MoveOut &M, const InstDev &I, int s, double x, double y) {
+  if (M.guard) {
+    double dx = x - M.xbuild[2 * s], dy = y - M.xbuild[2 * s + 1];
+    if (I.periodic) {
+      dx = dx > 0.5 * I.len_x ? dx - I.len_x : (dx < -0.5 * I.len_x ? dx + I.len_x : dx);
+      dy = dy > 0.5 * I.len_y ? dy - I.len_y : (dy < -0.5 * I.len_y ? dy + I.len_y : dy);
+    }
+    if (dx * dx + dy * dy > I.skin_half_sq) M.counters[kSkinExceeded] = 1;
+  }
+  if (!M.keys) return;
+  int cx, cy;
   if (!fluid_cell_xy(I, x, y, cx, cy)) {
     atomicAdd(&M.counters[kEscaped], 1);
     cx = cy = 0;
   }
-  if (!fluid_cell_xy(I, ox, oy, ocx, ocy)) ocx = ocy = 0;  // as binned: clamped to cell 0
   const int nx = I.nx, ny = I.ny;
-  int ddx = cx - ocx, ddy = cy - ocy;
+  const int oc = M.kslot[s] - I.cell_off;
+  int ddx = cx - oc % nx, ddy = cy - oc / nx;
   if (I.periodic) {
     ddx = ddx < 0 ? ddx + nx : ddx;
     ddy = ddy < 0 ? ddy + ny : ddy;

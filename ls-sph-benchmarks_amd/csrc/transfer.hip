@@ -1,6 +1,6 @@
 // transfer.hip -- host<->device transfers between the upload order and the
-// physical order, the neighbor-set dump used for validation, and the
-// steady-state copy of the diffusion test.
+// slot order of the records, the neighbor lists dumped for validation, and
+// the steady-state copy of the diffusion test.
 #include "pair_common.hpp"
 
 namespace lssph {
@@ -27,28 +27,77 @@ __global__ void k_nbr_count(int n, const Rec *__restrict__ mir, const int32_t *_
   counts_orig[orig_of[items[s]]] = cnt;
 }
 
+// Reference cell coordinates (0-based) of a position: see ref_cell_index.
+__device__ __forceinline__ void ref_cell_xy(const InstDev &I, double x, double y, bool wall, int &cx, int &cy) {
+  if (I.periodic) {
+    cx = (int)floor(x * I.inv_rcw_x);
+    cy = (int)floor(y * I.inv_rcw_y);
+    cx = (cx == I.rnx) ? I.rnx - 1 : cx;
+    cy = (cy == I.rny) ? I.rny - 1 : cy;
+  } else if (wall) {
+    cx = (int)ceil(x / I.h_eff) - 1;
+    cy = (int)ceil(y / I.h_eff) - 1;
+  } else {
+    cx = (int)ceil(x * I.inv_h_eff) - 1;
+    cy = (int)ceil(y * I.inv_h_eff) - 1;
+  }
+}
+
+// The neighbors of each fluid particle in the reference's summation order:
+// by position of the neighbor's reference cell in the 3x3 stencil of the
+// particle's reference cell (rows dy = -1..1, columns dx = -1..1), then by
+// upload index (fluid before walls, ascending).  The set is found on the
+// internal cell grid and sorted in place (validation only).
 __global__ void k_nbr_fill(int n, const Rec *__restrict__ mir, const int32_t *__restrict__ sinfo,
                            const int32_t *__restrict__ cstart, const int32_t *__restrict__ items,
                            const InstDev *__restrict__ inst, const int32_t *__restrict__ orig_of,
-                           const int64_t *__restrict__ offsets, int32_t *list) {
+                           const int64_t *__restrict__ offsets, int32_t *list, int64_t *sort_key) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n || info_ptype(sinfo[s]) != 0) return;
   const InstDev &I = inst[info_inst(sinfo[s])];
   const double2 me = load_xy(mir, s);
-  int64_t w = offsets[orig_of[items[s]]];
+  int rcx, rcy;
+  ref_cell_xy(I, me.x, me.y, false, rcx, rcy);
+  const int64_t w0 = offsets[orig_of[items[s]]];
+  int64_t w = w0;
   visit_cells(I, local_cell(I, me.x, me.y), cstart, [&](int t0, int t1, int code, int) {
     const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
     for (int t = t0; t < t1; ++t) {
       if (t == s) continue;
       const double2 xy = load_xy(mir, t);
-      if (sq_dist((xy.x + sx) - me.x, (xy.y + sy) - me.y) <= I.h_eff_sq) list[w++] = orig_of[items[t]];
+      if (sq_dist((xy.x + sx) - me.x, (xy.y + sy) - me.y) <= I.h_eff_sq) {
+        int jx, jy;
+        ref_cell_xy(I, xy.x, xy.y, info_ptype(sinfo[t]) != 0, jx, jy);
+        int dx = jx - rcx, dy = jy - rcy;
+        if (I.periodic) {
+          dx = dx > 1 ? dx - I.rnx : (dx < -1 ? dx + I.rnx : dx);
+          dy = dy > 1 ? dy - I.rny : (dy < -1 ? dy + I.rny : dy);
+        }
+        const int o = orig_of[items[t]];
+        list[w] = o;
+        sort_key[w] = ((int64_t)((dy + 1) * 3 + (dx + 1)) << 32) | (int64_t)o;
+        ++w;
+      }
     }
   });
+  for (int64_t i = w0 + 1; i < w; ++i) {
+    const int64_t k = sort_key[i];
+    const int32_t v = list[i];
+    int64_t j = i - 1;
+    while (j >= w0 && sort_key[j] > k) {
+      sort_key[j + 1] = sort_key[j];
+      list[j + 1] = list[j];
+      --j;
+    }
+    sort_key[j + 1] = k;
+    list[j + 1] = v;
+  }
 }
 
 // check_steady_state_mod.f90:50-67: flag per instance; instances that are
 // not steady store u in f_previous.
-__global__ void k_steady_copy(int n, const Rec *__restrict__ rec, const int32_t *__restrict__ info,
+__global__ void k_steady_copy(int n, const Rec *__restrict__ rec, const int32_t *__restrict__ slot_of,
+                              const int32_t *__restrict__ info,
                               const InstDev *__restrict__ inst, const unsigned long long *__restrict__ dfmax,
                               double *f_prev, int32_t *steady, int n_inst) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,17 +108,17 @@ __global__ void k_steady_copy(int n, const Rec *__restrict__ rec, const int32_t 
   if (p >= n || info_ptype(info[p]) != 0) return;
   const int ii = info_inst(info[p]);
   const double df = __longlong_as_double((long long)dfmax[ii]);
-  if (!(df < inst[ii].threshold)) f_prev[p] = rec[p].u;
+  if (!(df < inst[ii].threshold)) f_prev[p] = rec[slot_of[p]].u;
 }
 
-__global__ void k_download(int n, const Rec *__restrict__ rec, const int32_t *__restrict__ info,
-                           const int32_t *__restrict__ phys_of, const int32_t *__restrict__ key,
+__global__ void k_download(int n, const Rec *__restrict__ rec, const int32_t *__restrict__ slot_of,
+                           const int32_t *__restrict__ info, const int32_t *__restrict__ phys_of,
                            const InstDev *__restrict__ inst, double *stage, int32_t *ptype_out,
                            int32_t *cell_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int p = phys_of[i];
-  const Rec r = rec[p];
+  const Rec r = rec[slot_of[p]];
   stage[0 * (size_t)n + i] = r.x;
   stage[1 * (size_t)n + i] = r.y;
   stage[2 * (size_t)n + i] = r.u;
@@ -77,8 +126,9 @@ __global__ void k_download(int n, const Rec *__restrict__ rec, const int32_t *__
   stage[4 * (size_t)n + i] = r.rho;
   stage[5 * (size_t)n + i] = r.pre;
   stage[6 * (size_t)n + i] = r.tem;
-  ptype_out[i] = info_ptype(info[p]);
-  cell_out[i] = key[p] - inst[info_inst(info[p])].cell_off + 1;
+  const int pt = info_ptype(info[p]);
+  ptype_out[i] = pt;
+  cell_out[i] = ref_cell_index(inst[info_inst(info[p])], r.x, r.y, pt != 0);
 }
 
 struct RhsPtrs {
@@ -93,7 +143,8 @@ __global__ void k_download_rhs(int n_int, const int32_t *__restrict__ phys_of, R
   for (int f = 0; f < 4; ++f) stage[f * (size_t)n_int + i] = rhs.p[f][p];
 }
 
-// Records in upload order (physical = upload order until the first sort).
+// Records in upload order (slot = physical = upload order until the first
+// cell build).
 __global__ void k_upload(int n, const double *__restrict__ stage, const int32_t *__restrict__ ptype,
                          const int32_t *__restrict__ inst_of, const InstDev *__restrict__ inst, Rec *rec,
                          int32_t *info, int has_tem) {
@@ -115,27 +166,28 @@ __global__ void k_upload(int n, const double *__restrict__ stage, const int32_t 
 
 static inline int blocks(int n, int b) { return (n + b - 1) / b; }
 
-void launch_neighbor_count(hipStream_t s, Device &d, int, int32_t *counts_orig) {
-  hipLaunchKernelGGL(k_nbr_count, dim3(blocks(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.mirror, d.sinfo,
+void launch_neighbor_count(hipStream_t s, Device &d, int cur, int32_t *counts_orig) {
+  hipLaunchKernelGGL(k_nbr_count, dim3(blocks(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.mir[cur], d.sinfo,
                      d.cell_start, d.items, d.inst, d.orig_of, counts_orig);
 }
 
-void launch_neighbor_fill(hipStream_t s, Device &d, int, const int64_t *offsets_orig, int32_t *list) {
-  hipLaunchKernelGGL(k_nbr_fill, dim3(blocks(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.mirror, d.sinfo,
-                     d.cell_start, d.items, d.inst, d.orig_of, offsets_orig, list);
+void launch_neighbor_fill(hipStream_t s, Device &d, int cur, const int64_t *offsets_orig, int32_t *list,
+                          int64_t *sort_key) {
+  hipLaunchKernelGGL(k_nbr_fill, dim3(blocks(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.mir[cur], d.sinfo,
+                     d.cell_start, d.items, d.inst, d.orig_of, offsets_orig, list, sort_key);
 }
 
 void launch_steady(hipStream_t s, Device &d, int cur) {
   launch_dfmax(s, d, cur);
   const int n = d.n_total > d.n_inst ? d.n_total : d.n_inst;
-  hipLaunchKernelGGL(k_steady_copy, dim3(blocks(n, 256)), dim3(256), 0, s, d.n_total, d.rec[cur], d.info,
-                     d.inst, d.dfmax, d.f_prev, d.steady, d.n_inst);
+  hipLaunchKernelGGL(k_steady_copy, dim3(blocks(n, 256)), dim3(256), 0, s, d.n_total, d.mir[cur], d.slot_of,
+                     d.info, d.inst, d.dfmax, d.f_prev, d.steady, d.n_inst);
 }
 
 void launch_download(hipStream_t s, Device &d, int cur) {
   const int n = d.n_total;
-  hipLaunchKernelGGL(k_download, dim3(blocks(n, 256)), dim3(256), 0, s, n, d.rec[cur], d.info, d.phys_of, d.key,
-                     d.inst, d.stage_f64, d.stage_i32, d.perm_tmp);
+  hipLaunchKernelGGL(k_download, dim3(blocks(n, 256)), dim3(256), 0, s, n, d.mir[cur], d.slot_of, d.info,
+                     d.phys_of, d.inst, d.stage_f64, d.stage_i32, d.perm_tmp);
 }
 
 void launch_download_rhs(hipStream_t s, Device &d) {
@@ -149,7 +201,7 @@ void launch_upload(hipStream_t s, Device &d, const double *stage, const int32_t 
                    int has_tem) {
   const int n = d.n_total;
   hipLaunchKernelGGL(k_upload, dim3(blocks(n, 256)), dim3(256), 0, s, n, stage, ptype, inst_of, d.inst,
-                     d.rec[0], d.info, has_tem);
+                     d.mir[0], d.info, has_tem);
 }
 
 }  // namespace lssph

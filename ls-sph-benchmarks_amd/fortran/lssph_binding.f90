@@ -148,6 +148,11 @@ module lssph_binding
             type(c_ptr), value :: h
             integer(c_int32_t), value :: block, list_len, reorder_interval
         end function
+        integer(c_int32_t) function lssph_set_skin(h, skin) bind(C, name="lssph_set_skin")
+            import :: c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: skin
+        end function
         type(c_ptr) function lssph_stream(h) bind(C, name="lssph_stream")
             import :: c_ptr
             type(c_ptr), value :: h

@@ -51,7 +51,7 @@ EXPORTS = ("lssph_create", "lssph_destroy", "lssph_last_error", "lssph_upload", 
            "lssph_download_rhs", "lssph_rk2", "lssph_rk4", "lssph_pst", "lssph_step",
            "lssph_check_steady_state", "lssph_download_vm", "lssph_download_cell_index",
            "lssph_neighbors", "lssph_stats", "lssph_reset_stats", "lssph_synchronize",
-           "lssph_set_tuning", "lssph_stream", "lssph_profile", "lssph_kernel_times")
+           "lssph_set_tuning", "lssph_set_skin", "lssph_stream", "lssph_profile", "lssph_kernel_times")
 
 
 def load_library(path: Path | None = None):
@@ -83,6 +83,7 @@ def load_library(path: Path | None = None):
     lib.lssph_neighbors.argtypes = [vp, _I, _I, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     lib.lssph_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     lib.lssph_set_tuning.argtypes = [vp, i32, i32, i32]
+    lib.lssph_set_skin.argtypes = [vp, ctypes.c_double]
     lib.lssph_stream.restype = vp
     lib.lssph_stream.argtypes = [vp]
     lib.lssph_profile.argtypes = [vp, i32]
@@ -110,7 +111,8 @@ class Simulation:
     states: matching particle States.  Upload order: the fluid blocks of all
     instances, then their wall blocks."""
 
-    def __init__(self, params, num_int: Sequence[int], num_ext: Sequence[int], precision: int | None = None):
+    def __init__(self, params, num_int: Sequence[int], num_ext: Sequence[int], precision: int | None = None,
+                 skin: float | None = None):
         self.lib = load_library()
         plist = params if isinstance(params, (list, tuple)) else [params]
         cparams = [p.to_c(precision) if isinstance(p, Param) else p for p in plist]
@@ -133,6 +135,8 @@ class Simulation:
         if rc != 0:
             raise LssphError(rc, self.lib.lssph_last_error(None).decode())
         self.h = h
+        if skin is not None:
+            self.set_skin(skin)
 
     # -- lifecycle ------------------------------------------------------------
     def close(self):
@@ -263,6 +267,10 @@ class Simulation:
     def set_tuning(self, block: int = 0, list_len: int = 0, reorder_interval: int = 0):
         self._check(self.lib.lssph_set_tuning(self.h, block, list_len, reorder_interval))
 
+    def set_skin(self, skin: float):
+        """Cell-list skin as a fraction of h_eff (0: rebuild after every pass)."""
+        self._check(self.lib.lssph_set_skin(self.h, ctypes.c_double(skin)))
+
     def profile(self, enable: bool = True):
         self._check(self.lib.lssph_profile(self.h, 1 if enable else 0))
 
@@ -307,9 +315,10 @@ def split_states(arrs: dict, num_int: Sequence[int], num_ext: Sequence[int]) -> 
     return out
 
 
-def simulation_from_state(param: Param, st: State, precision: int | None = None) -> Simulation:
+def simulation_from_state(param: Param, st: State, precision: int | None = None,
+                          skin: float | None = None) -> Simulation:
     """Creates a handle for one box, uploads st and builds the cell list."""
-    sim = Simulation(param, [st.num_int], [st.num_ext], precision=precision)
+    sim = Simulation(param, [st.num_int], [st.num_ext], precision=precision, skin=skin)
     sim.upload(st)
     sim.init_cell()
     return sim

@@ -156,15 +156,18 @@ def test_incremental_cell_builds_keep_reference_order(case):
 @pytest.mark.parametrize("rk", [2, 4])
 def test_far_moves_between_builds(rk):
     """A uniform flow fast enough to carry particles one to three cells per
-    stage: the incremental build must take those particles from the stray
-    list (and select members one by one in cells with many arrivals); the
-    states and neighbor lists still equal the oracle's."""
+    stage, with the cells rebuilt after every pass (skin 0): the incremental
+    build must take those particles from the stray list (and select members
+    one by one in cells with many arrivals); the states and neighbor lists
+    still equal the oracle's."""
     param, st = workloads.taylor_green_periodic(n=24, re=100.0, rk=rk, jitter=0.3, seed=23)
     cw = 1.0 / int(1.0 / param.h_eff)
     speed = 3.0 * cw / param.dt
     st.u[:] = speed
     st.v[:] = 0.7 * speed
-    o, s = _pair(param, st)
+    o = OracleSim(param.to_c(), st.num_int, st.num_ext)
+    o.init(st)
+    s = simulation_from_state(param, st, skin=0.0)
     with s:
         for _ in range(2):
             o.step(1)
@@ -172,5 +175,35 @@ def test_far_moves_between_builds(rk):
             co, lo = o.neighbors()
             cs, ls = s.neighbors()
             assert np.array_equal(co, cs) and np.array_equal(lo, ls)
+        e = _cmp(s.download(), o.get_state(), ("x", "y", "u", "v", "rho", "pre"))
+        assert max(e.values()) <= TOL, e
+
+
+def test_skin_guard_reports_moves_beyond_the_skin():
+    """With the default skin, the same fast flow moves particles farther than
+    half the skin within a step: the library reports it instead of using a
+    stale cell list."""
+    from lssph.engine import LssphError
+    param, st = workloads.taylor_green_periodic(n=24, re=100.0, rk=4, jitter=0.3, seed=23)
+    speed = 3.0 / int(1.0 / param.h_eff) / param.dt
+    st.u[:] = speed
+    st.v[:] = 0.7 * speed
+    with simulation_from_state(param, st) as s:
+        s.step(1)
+        with pytest.raises(LssphError, match="skin"):
+            s.synchronize()
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.3])
+def test_results_do_not_depend_on_the_skin(skin):
+    """The cell-list skin only changes which candidates are tested; the
+    trajectory matches the oracle either way."""
+    param, st = workloads.taylor_green_periodic(n=40, re=100.0, rk=4, jitter=0.2, seed=31)
+    o = OracleSim(param.to_c(), st.num_int, st.num_ext)
+    o.init(st)
+    with simulation_from_state(param, st, skin=skin) as s:
+        for _ in range(3):
+            o.step(1)
+            s.step(1)
         e = _cmp(s.download(), o.get_state(), ("x", "y", "u", "v", "rho", "pre"))
         assert max(e.values()) <= TOL, e
