@@ -172,8 +172,7 @@ InstDev make_inst(const lssph_params_t &p, double skin) {
   I.y_max = p.len_y + p.wl_thick;
   I.periodic = p.periodic;
   const double w = cell_width(p, skin);
-  const double sk = w - p.h_eff;  // the skin this instance gets
-  I.skin_half_sq = 0.25 * sk * sk;
+  double sk = w - p.h_eff;  // the skin this instance gets
   if (p.periodic) {
     I.nx = (int)std::floor(p.len_x / w);
     I.ny = (int)std::floor(p.len_y / w);
@@ -183,6 +182,8 @@ InstDev make_inst(const lssph_params_t &p, double skin) {
     I.rny = (int)std::floor(p.len_y / p.h_eff);
     I.inv_rcw_x = (double)I.rnx / p.len_x;
     I.inv_rcw_y = (double)I.rny / p.len_y;
+    // whole cells per period: the cells are at least w wide
+    sk = std::min(p.len_x / I.nx, p.len_y / I.ny) - p.h_eff;
   } else {
     I.nx = (int)std::floor((p.len_x + 2.0 * p.wl_thick) / w) + 1;
     I.ny = (int)std::floor((p.len_y + 2.0 * p.wl_thick) / w) + 1;
@@ -192,6 +193,10 @@ InstDev make_inst(const lssph_params_t &p, double skin) {
     I.rnx = (int)std::ceil((p.len_x + 2.0 * p.wl_thick) / p.h_eff);
     I.rny = (int)std::ceil((p.len_y + 2.0 * p.wl_thick) / p.h_eff);
   }
+  // each particle may move 0.45 skin between builds, which leaves a tenth
+  // of the skin as margin for rounding in the binning; skin 0 asks for a
+  // build after every pass whatever width the cells have
+  I.skin_half_sq = skin > 0.0 ? (0.45 * sk) * (0.45 * sk) : 0.0;
   return I;
 }
 

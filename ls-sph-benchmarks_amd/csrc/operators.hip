@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   const InstDev &I = A.inst[info_inst(inf)];
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
-  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, s, me.x, me.y, lds, A.list_len, B, tid);
+  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, s, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
   if (cnt > A.list_len) {
     atomicAdd(&A.stats64[kOverflows], 1ull);
     atomicMax(&A.counters[kMaxNbr], cnt);
@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
   const int kt = A.kt;
-  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, s, me.x, me.y, lds, A.list_len, B, tid);
+  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, s, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
   if (cnt > A.list_len) atomicAdd(&A.stats64[kOverflows], 1ull);
 
   LsSums<R, NF> S;

@@ -111,16 +111,31 @@ __device__ __forceinline__ bool wave_wraps(const InstDev &I, int lc) {
   return __ballot(stencil_wraps(I, lc)) != 0ull;
 }
 
-// Candidate scan of slot s: fills the LDS list (slot | code << kSlotBits),
-// returns the neighbor count.  Two candidates per iteration, so two record
+// Periodic image of x_j seen from x_i: the shift (-L, 0 or +L) that brings
+// x_j within L/2 of x_i.  Between cell builds particles may have wrapped
+// around the box, so the shift is taken per pair from the positions, and
+// offsets are formed as (x_j + shift) - x_i (the oracle's expression; for
+// pairs within h_eff < L/3 its cell-stencil shift is the same).
+__device__ __forceinline__ double image_shift(double xj, double xi, double len) {
+  const double d = xj - xi;
+  return d > 0.5 * len ? -len : (d < -0.5 * len ? len : 0.0);
+}
+
+__device__ __forceinline__ int image_code(double sx, double sy) {
+  return (sx < 0.0 ? 1 : (sx > 0.0 ? 2 : 0)) | ((sy < 0.0 ? 1 : (sy > 0.0 ? 2 : 0)) << 2);
+}
+
+// Candidate scan of slot s within radius^2 r2: fills the LDS list (slot |
+// image code << kSlotBits), returns the count.  Waves in which no lane's
+// stencil wraps skip the image arithmetic (x_j + 0 == x_j, so both paths
+// form bit-identical offsets).  Two candidates per iteration, so two record
 // loads are in flight before the first test.
 __device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
-                                               const InstDev &I, int lc, int s, double xi, double yi,
+                                               const InstDev &I, int lc, int s, double xi, double yi, double r2,
                                                int32_t *lds, int list_len, int B, int tid) {
   int cnt = 0;
-  const double he2 = I.h_eff_sq;
   auto accept = [&](int t, double dx, double dy, int code) {
-    if (sq_dist(dx, dy) <= he2 && t != s) {
+    if (sq_dist(dx, dy) <= r2 && t != s) {
       if (cnt < list_len) lds[cnt * B + tid] = t | (code << kSlotBits);
       ++cnt;
     }
@@ -139,57 +154,67 @@ __device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, cons
       }
     });
   } else {
-    visit_cells(I, lc, cstart, [&](int t0, int t1, int code, int) {
-      const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
+    auto test = [&](int t, double2 p) {
+      const double sx = image_shift(p.x, xi, I.len_x), sy = image_shift(p.y, yi, I.len_y);
+      accept(t, (p.x + sx) - xi, (p.y + sy) - yi, image_code(sx, sy));
+    };
+    visit_cells(I, lc, cstart, [&](int t0, int t1, int, int) {
       int t = t0;
       for (; t + 2 <= t1; t += 2) {
         const double2 p0 = load_xy(mir, t), p1 = load_xy(mir, t + 1);
-        accept(t, (p0.x + sx) - xi, (p0.y + sy) - yi, code);
-        accept(t + 1, (p1.x + sx) - xi, (p1.y + sy) - yi, code);
+        test(t, p0);
+        test(t + 1, p1);
       }
-      if (t < t1) {
-        const double2 p0 = load_xy(mir, t);
-        accept(t, (p0.x + sx) - xi, (p0.y + sy) - yi, code);
-      }
+      if (t < t1) test(t, load_xy(mir, t));
     });
   }
   return cnt;
 }
 
+// Calls f(nb_record, dx, dy) for every neighbor within h_eff of slot s, in
+// stencil order, by a walk over the candidates of the cell list.
+template <typename F>
+__device__ __forceinline__ void walk_stencil(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
+                                             const InstDev &I, int lc, int s, const Rec &me, F &&f) {
+  const bool per = I.periodic;
+  visit_cells(I, lc, cstart, [&](int t0, int t1, int, int) {
+    for (int t = t0; t < t1; ++t) {
+      if (t == s) continue;
+      const double2 xy = load_xy(mir, t);
+      const double sx = per ? image_shift(xy.x, me.x, I.len_x) : 0.0;
+      const double sy = per ? image_shift(xy.y, me.y, I.len_y) : 0.0;
+      const double dx = (xy.x + sx) - me.x, dy = (xy.y + sy) - me.y;
+      if (sq_dist(dx, dy) <= I.h_eff_sq) f(mir[t], dx, dy);
+    }
+  });
+}
+
 // Calls f(nb_record, dx, dy) for every accepted neighbor of slot s in
-// summation order: from the LDS list, or (overflow, or list_len == 0) by a
-// direct walk over the candidates.
+// summation order: from the LDS list filled by scan_candidates, or (the
+// list overflowed) by a walk over the candidates.
 template <typename F>
 __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
                                                const InstDev &I, int lc, int s, const Rec &me,
                                                const int32_t *lds, int cnt, int list_len, int B, int tid,
                                                F &&f) {
-  if (cnt <= list_len) {
-    if (!wave_wraps(I, lc)) {
-      for (int k = 0; k < cnt; ++k) {
-        const Rec nb = mir[lds[k * B + tid] & kSlotMask];
-        f(nb, nb.x - me.x, nb.y - me.y);
-      }
-    } else {
-      for (int k = 0; k < cnt; ++k) {
-        const int e = lds[k * B + tid];
-        const int t = e & kSlotMask, code = e >> kSlotBits;
-        const Rec nb = mir[t];
-        const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
-        const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
-        f(nb, dx, dy);
-      }
+  if (cnt > list_len) {
+    walk_stencil(mir, cstart, I, lc, s, me, f);
+    return;
+  }
+  if (!wave_wraps(I, lc)) {
+    for (int k = 0; k < cnt; ++k) {
+      const Rec nb = mir[lds[k * B + tid] & kSlotMask];
+      f(nb, nb.x - me.x, nb.y - me.y);
     }
   } else {
-    visit_cells(I, lc, cstart, [&](int t0, int t1, int code, int) {
-      const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
-      for (int t = t0; t < t1; ++t) {
-        if (t == s) continue;
-        const double2 xy = load_xy(mir, t);
-        const double dx = (xy.x + sx) - me.x, dy = (xy.y + sy) - me.y;
-        if (sq_dist(dx, dy) <= I.h_eff_sq) f(mir[t], dx, dy);
-      }
-    });
+    for (int k = 0; k < cnt; ++k) {
+      const int e = lds[k * B + tid];
+      const int t = e & kSlotMask, code = e >> kSlotBits;
+      const Rec nb = mir[t];
+      const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
+      const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
+      f(nb, dx, dy);
+    }
   }
 }
 

@@ -16,11 +16,12 @@ __global__ void k_nbr_count(int n, const Rec *__restrict__ mir, const int32_t *_
   const InstDev &I = inst[info_inst(sinfo[s])];
   const double2 me = load_xy(mir, s);
   int cnt = 0;
-  visit_cells(I, local_cell(I, me.x, me.y), cstart, [&](int t0, int t1, int code, int) {
-    const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
+  visit_cells(I, local_cell(I, me.x, me.y), cstart, [&](int t0, int t1, int, int) {
     for (int t = t0; t < t1; ++t) {
       if (t == s) continue;
       const double2 xy = load_xy(mir, t);
+      const double sx = I.periodic ? image_shift(xy.x, me.x, I.len_x) : 0.0;
+      const double sy = I.periodic ? image_shift(xy.y, me.y, I.len_y) : 0.0;
       if (sq_dist((xy.x + sx) - me.x, (xy.y + sy) - me.y) <= I.h_eff_sq) ++cnt;
     }
   });
@@ -60,11 +61,12 @@ __global__ void k_nbr_fill(int n, const Rec *__restrict__ mir, const int32_t *__
   ref_cell_xy(I, me.x, me.y, false, rcx, rcy);
   const int64_t w0 = offsets[orig_of[items[s]]];
   int64_t w = w0;
-  visit_cells(I, local_cell(I, me.x, me.y), cstart, [&](int t0, int t1, int code, int) {
-    const double sx = shift_of(code & 3, I.len_x), sy = shift_of(code >> 2, I.len_y);
+  visit_cells(I, local_cell(I, me.x, me.y), cstart, [&](int t0, int t1, int, int) {
     for (int t = t0; t < t1; ++t) {
       if (t == s) continue;
       const double2 xy = load_xy(mir, t);
+      const double sx = I.periodic ? image_shift(xy.x, me.x, I.len_x) : 0.0;
+      const double sy = I.periodic ? image_shift(xy.y, me.y, I.len_y) : 0.0;
       if (sq_dist((xy.x + sx) - me.x, (xy.y + sy) - me.y) <= I.h_eff_sq) {
         int jx, jy;
         ref_cell_xy(I, xy.x, xy.y, info_ptype(sinfo[t]) != 0, jx, jy);

@@ -179,6 +179,34 @@ def test_far_moves_between_builds(rk):
         assert max(e.values()) <= TOL, e
 
 
+@pytest.mark.parametrize("rk", [2, 4])
+def test_periodic_wraps_between_builds(rk):
+    """A uniform diagonal flow at 0.3 of the skin per step carries particles
+    across both periodic edges while the cell list built at the start of the
+    step is still in use: the image shift of each pair is taken from the
+    positions, so states and neighbor lists stay the oracle's."""
+    param, st = workloads.taylor_green_periodic(n=24, re=100.0, rk=rk, jitter=0.3, seed=29)
+    st.x[:] = np.mod(st.x + 0.018, 1.0)  # a column of particles near each edge
+    st.y[:] = np.mod(st.y + 0.017, 1.0)
+    step = 0.3 * 0.08 * param.h_eff
+    st.u[:] = 0.8 * step / param.dt
+    st.v[:] = 0.6 * step / param.dt
+    o = OracleSim(param.to_c(), st.num_int, st.num_ext)
+    o.init(st)
+    with simulation_from_state(param, st) as s:
+        for _ in range(6):
+            o.step(1)
+            s.step(1)
+            co, lo = o.neighbors()
+            cs, ls = s.neighbors()
+            assert np.array_equal(co, cs) and np.array_equal(lo, ls)
+        got = s.download()
+        n = st.num_int
+        assert np.any(got["x"][:n] < st.x[:n] - 0.5) and np.any(got["y"][:n] < st.y[:n] - 0.5)
+        e = _cmp(got, o.get_state(), ("x", "y", "u", "v", "rho", "pre"))
+        assert max(e.values()) <= TOL, e
+
+
 def test_skin_guard_reports_moves_beyond_the_skin():
     """With the default skin, the same fast flow moves particles farther than
     half the skin within a step: the library reports it instead of using a
