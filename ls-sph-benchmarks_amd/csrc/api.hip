@@ -837,7 +837,7 @@ int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len, int3
   }
   if (list_len) {
     if (list_len < 8 || list_len > 128) return fail(h, LSSPH_E_ARG, "list_len must be in [8, 128]");
-    if ((size_t)list_len * (block ? block : h->d.block) * 4 > 64 * 1024)
+    if ((size_t)(list_len + 1) * (block ? block : h->d.block) * 4 > 64 * 1024)
       return fail(h, LSSPH_E_ARG, "list_len * block exceeds 64 KiB of LDS");
     h->d.list_len = list_len;
   }

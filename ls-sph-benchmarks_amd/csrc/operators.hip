@@ -417,17 +417,18 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   const InstDev &I = A.inst[info_inst(inf)];
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
-  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, s, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
+  // list entries, the particle itself included
+  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
   if (cnt > A.list_len) {
     atomicAdd(&A.stats64[kOverflows], 1ull);
-    atomicMax(&A.counters[kMaxNbr], cnt);
+    atomicMax(&A.counters[kMaxNbr], cnt - 1);
   }
   double d[NF][5];
 #pragma unroll
   for (int f = 0; f < NF; ++f)
 #pragma unroll
     for (int i = 0; i < 5; ++i) d[f][i] = 0.0;
-  if (cnt >= 5) {  // LSSPH_A_mod.f90:208 (num_neighbors >= q)
+  if (cnt - 1 >= 5) {  // LSSPH_A_mod.f90:208 (num_neighbors >= q)
     if constexpr (MIXED) {
       if (!derivs32<CLS, DELTA, KT>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d)) {
         const int k = atomicAdd(A.fb_count, 1);
@@ -497,7 +498,8 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
   const int kt = A.kt;
-  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, s, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
+  // list entries, the particle itself included
+  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
   if (cnt > A.list_len) atomicAdd(&A.stats64[kOverflows], 1ull);
 
   LsSums<R, NF> S;
@@ -518,8 +520,9 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
                      const double dwdr = I.w_norm * kshape_dw<double, KT>(kt, q) * I.dw_fac;
                      const double vj = nb.vol;
                      const double inv_r = 1.0 / r;
-                     const double dwx = dwdr * (-dx) * inv_r * vj;
-                     const double dwy = dwdr * (-dy) * inv_r * vj;
+                     // the particle itself (r = 0) adds no shift
+                     const double dwx = r > 0.0 ? dwdr * (-dx) * inv_r * vj : 0.0;
+                     const double dwy = r > 0.0 ? dwdr * (-dy) * inv_r * vj : 0.0;
                      const double wr = w * inv_w_ave;
                      const double f = 1.0 + 0.2 * ((wr * wr) * (wr * wr));
                      sx = sx + f * dwx;
@@ -533,7 +536,8 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
                      const float shp = kshape_w<float, KT>(kt, q);
                      const float vwn = float(nb.vol * I.w_norm);
                      // dW/dr * (-x/r) * V = V * w_norm * dw_fac * shape'(q) * (-xs / q)
-                     const float g = float(nb.vol * I.w_norm * I.dw_fac) * kshape_dw<float, KT>(kt, q) * fast_rcpf(q);
+                     const float g0 = float(nb.vol * I.w_norm * I.dw_fac) * kshape_dw<float, KT>(kt, q) * fast_rcpf(q);
+                     const float g = q > 0.0f ? g0 : 0.0f;  // the particle itself adds no shift
                      const float wr = float(I.w_norm * inv_w_ave) * shp;
                      const float f = 1.0f + 0.2f * ((wr * wr) * (wr * wr));
                      sx = sx + f * (g * -xs);
@@ -554,7 +558,7 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   }
   // LS correction of u, v, rho (, tem) to the shifted position (:352-415)
   double X[NF][5];
-  if (cnt >= 5) {
+  if (cnt - 1 >= 5) {  // shift_interpolate_mod.f90:240 (num_neighbors >= q)
     bool solved = false;
     if constexpr (MIXED) {
       solved = ldlt_solve_refine<NF>(S, X, I.cond_max);
@@ -648,8 +652,9 @@ __global__ void __launch_bounds__(256) k_dfmax(const Rec *__restrict__ rec, cons
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-// LDS of the neighbor lists: one 32-bit entry per neighbor.
-static size_t list_lds_bytes(int list_len, int block) { return (size_t)4 * list_len * block; }
+// LDS of the neighbor lists: list_len 32-bit entries per thread plus the
+// trash entry of the scan.
+static size_t list_lds_bytes(int list_len, int block) { return (size_t)4 * (list_len + 1) * block; }
 
 template <int CLS, bool MIXED, bool DELTA, int KT>
 static void lsa_launch(hipStream_t s, const LsaArgs &a, int block, size_t lds) {

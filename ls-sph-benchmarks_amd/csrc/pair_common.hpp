@@ -125,30 +125,41 @@ __device__ __forceinline__ int image_code(double sx, double sy) {
   return (sx < 0.0 ? 1 : (sx > 0.0 ? 2 : 0)) | ((sy < 0.0 ? 1 : (sy > 0.0 ? 2 : 0)) << 2);
 }
 
-// Candidate scan of slot s within radius^2 r2: fills the LDS list (slot |
-// image code << kSlotBits), returns the count.  Waves in which no lane's
-// stencil wraps skip the image arithmetic (x_j + 0 == x_j, so both paths
-// form bit-identical offsets).  Two candidates per iteration, so two record
-// loads are in flight before the first test.
+// Candidate scan of the particle at (xi, yi) within radius^2 r2: fills the
+// LDS list (slot | image code << kSlotBits) in stencil order and returns the
+// count.  The particle's own slot is among the candidates and is accepted
+// (r = 0): it takes one list entry, and a pair operator sees it as a pair
+// with zero offset (the LS basis vanishes there, so it adds exactly zero to
+// the sums).  The loop body has no branch: every candidate is written at the
+// current list position (position list_len is a trash entry) and the
+// position advances when the candidate is accepted.  Waves in which no
+// lane's stencil wraps skip the image arithmetic (x_j + 0 == x_j, so both
+// paths form bit-identical offsets).  Four candidates per iteration (two in
+// the wrapping waves), so several record loads are in flight before the
+// first test.
 __device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
-                                               const InstDev &I, int lc, int s, double xi, double yi, double r2,
+                                               const InstDev &I, int lc, double xi, double yi, double r2,
                                                int32_t *lds, int list_len, int B, int tid) {
   int cnt = 0;
+  char *lbase = reinterpret_cast<char *>(lds) + 4 * tid;
+  const unsigned B4 = 4u * (unsigned)B;
   auto accept = [&](int t, double dx, double dy, int code) {
-    if (sq_dist(dx, dy) <= r2 && t != s) {
-      if (cnt < list_len) lds[cnt * B + tid] = t | (code << kSlotBits);
-      ++cnt;
-    }
+    const unsigned k = min((unsigned)cnt, (unsigned)list_len);
+    *reinterpret_cast<int32_t *>(lbase + __umul24(k, B4)) = t | (code << kSlotBits);
+    cnt += (sq_dist(dx, dy) <= r2) ? 1 : 0;
   };
   if (!wave_wraps(I, lc)) {
     visit_cells(I, lc, cstart, [&](int t0, int t1, int, int) {
       int t = t0;
-      for (; t + 2 <= t1; t += 2) {
+      for (; t + 4 <= t1; t += 4) {
         const double2 p0 = load_xy(mir, t), p1 = load_xy(mir, t + 1);
+        const double2 p2 = load_xy(mir, t + 2), p3 = load_xy(mir, t + 3);
         accept(t, p0.x - xi, p0.y - yi, 0);
         accept(t + 1, p1.x - xi, p1.y - yi, 0);
+        accept(t + 2, p2.x - xi, p2.y - yi, 0);
+        accept(t + 3, p3.x - xi, p3.y - yi, 0);
       }
-      if (t < t1) {
+      for (; t < t1; ++t) {
         const double2 p0 = load_xy(mir, t);
         accept(t, p0.x - xi, p0.y - yi, 0);
       }
@@ -190,8 +201,9 @@ __device__ __forceinline__ void walk_stencil(const Rec *__restrict__ mir, const 
 }
 
 // Calls f(nb_record, dx, dy) for every accepted neighbor of slot s in
-// summation order: from the LDS list filled by scan_candidates, or (the
-// list overflowed) by a walk over the candidates.
+// summation order: from the LDS list filled by scan_candidates, which holds
+// the particle itself as a pair with dx = dy = 0, or (the list overflowed)
+// by a walk over the candidates, which skips it.
 template <typename F>
 __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
                                                const InstDev &I, int lc, int s, const Rec &me,
