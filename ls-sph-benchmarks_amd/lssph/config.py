@@ -412,7 +412,5 @@ def check_param(cfg: Config) -> None:
         fail("PST_C and COE_DELTA_SPH must be non-negative.")
     if abs(cfg.len_x / float(cfg.num_x) - cfg.len_y / float(cfg.num_y)) > 1.0e-8:
         fail("Particle spacing must be isotropic (dx == dy).")
-    if cfg.sph_model != 3:
-        fail("SPH_MODEL 3 (LS-SPH, 2nd-order Taylor expansion) is the operator this package runs.")
     if cfg.precision not in (0, 1):
         fail("PRECISION must be 0 (FP64) or 1 (mixed).")

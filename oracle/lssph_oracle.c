@@ -12,7 +12,13 @@
  *     escaped-particle check, which the reference does not have;
  *   - cell lists are rebuilt with a deterministic counting sort whose
  *     within-cell order equals the reference's single-thread order
- *     (fluid in ascending index, then walls in ascending index).
+ *     (fluid in ascending index, then walls in ascending index);
+ *   - SPH_MODEL 5 shifting uses the q = 14 basis: the reference's writes
+ *     a_vec(15:20) past the end of its q = 14 array
+ *     (shift_interpolate_mod.f90:174, guard ">= 5" where ">= 6" is meant),
+ *     which flang rejects at compile time.
+ * Operators: LS type A for SPH_MODEL 3-6 (q = 5, 9, 14, 20) and classical
+ * SPH for SPH_MODEL 1-2; shifting with q = 2 for SPH_MODEL 1-2.
  * Per-particle loops are OpenMP-parallel; each iteration writes only its
  * own particle, so results do not depend on the thread count.
  */
@@ -246,6 +252,8 @@ static inline void basis_a(int q, double x, double y, const double *ih, double *
 
 static int q_of_model(int sph_model, int *k) {
   switch (sph_model) {
+    case 1:
+    case 2: *k = 1; return 2; /* shifting only, shift_interpolate_mod.f90:43-44 */
     case 4: *k = 3; return 9;
     case 5: *k = 4; return 14;
     case 6: *k = 5; return 20;
@@ -551,11 +559,82 @@ static void lssph_a(orc_sim *s, int me) {
   s->rhs_v[me] = r_v;
 }
 
-/* calculate_RHS_mod.f90:36-50 */
+/* ------------------------------------------------------------------------ */
+/* classical_SPH (classical_SPH_mod.f90:26-170) for particle me: kernel      */
+/* gradient sums with x_ij = x_i - x_j; SPH_MODEL 1 sums the pressures in    */
+/* the gradient (:115-117), SPH_MODEL 2 differences them (:118-120).        */
+/* ------------------------------------------------------------------------ */
+static void classical_sph(orc_sim *s, int me) {
+  const lssph_params_t *p = &s->p;
+  const int sum_model = (p->sph_model == 1);
+  double h = p->h, h_eff_sq = p->h_eff * p->h_eff, m = p->sp_mass;
+  double my_x = s->x[me], my_y = s->y[me], my_u = s->u[me], my_v = s->v[me];
+  double my_rho = s->rho[me], my_pre = s->pre[me], my_tem = s->has_tem ? s->tem[me] : 0.0;
+  double du_dx = 0.0, dv_dy = 0.0, dp_dx = 0.0, dp_dy = 0.0;
+  double lap_u = 0.0, lap_v = 0.0, lap_tem = 0.0, lap_rho = 0.0;
+  int c0 = s->cell_of[me];
+  for (int nc = 0; nc < 9; ++nc) {
+    int c = s->nbr[9 * c0 + nc];
+    if (c < 0) continue;
+    double sx = s->nbr_sx[9 * c0 + nc], sy = s->nbr_sy[9 * c0 + nc];
+    for (int t = s->cell_start[c]; t < s->cell_start[c + 1]; ++t) {
+      int j = s->cell_list[t];
+      if (j == me) continue;
+      double x_ij = my_x - (s->x[j] + sx), y_ij = my_y - (s->y[j] + sy);
+      double r2 = x_ij * x_ij + y_ij * y_ij;
+      if (r2 > h_eff_sq) continue;
+      double r = sqrt(r2);
+      double dwdr = orc_kernel_dw(p->kernel_type, r, h);
+      double vj = m / s->rho[j];
+      double dwx = dwdr * x_ij / r * vj;
+      double dwy = dwdr * y_ij / r * vj;
+      du_dx = du_dx + dwx * (s->u[j] - my_u);
+      dv_dy = dv_dy + dwy * (s->v[j] - my_v);
+      double pp = sum_model ? s->pre[j] + my_pre : s->pre[j] - my_pre;
+      dp_dx = dp_dx + dwx * pp;
+      dp_dy = dp_dy + dwy * pp;
+      double g = 2.0 * vj * dwdr / r;
+      lap_u = lap_u + g * (my_u - s->u[j]);
+      lap_v = lap_v + g * (my_v - s->v[j]);
+      if (s->has_tem) lap_tem = lap_tem + g * (my_tem - s->tem[j]);
+      lap_rho = lap_rho + g * (my_rho - s->rho[j]);
+    }
+  }
+  /* :139-168 */
+  if (p->target_problem == 1) {
+    s->rhs_u[me] = p->diff_coef * lap_u +
+                   p->diff_source * (2.0 * ORC_PI * ORC_PI * sin(ORC_PI * (my_x - p->wl_thick)) *
+                                     cos(ORC_PI * (my_y - p->wl_thick)));
+    return;
+  }
+  double delta = p->coe_delta_sph * p->sound_speed * h * p->zeta_rsst;
+  double r_rho = -my_rho * (du_dx + dv_dy) + delta * lap_rho;
+  double r_u = -dp_dx / my_rho + (p->vis_ref / my_rho) * lap_u;
+  double r_v = -dp_dy / my_rho + (p->vis_ref / my_rho) * lap_v;
+  r_rho = r_rho / (p->zeta_rsst * p->zeta_rsst);
+  if (p->target_problem == 4) {
+    r_v = r_v + p->alpha_ref * (my_tem - p->tem_ave) * p->gravity;
+    r_u = r_u / (p->xi_vim * p->xi_vim);
+    r_v = r_v / (p->xi_vim * p->xi_vim);
+    s->rhs_tem[me] = p->thermal_dif * lap_tem;
+  }
+  s->rhs_rho[me] = r_rho;
+  s->rhs_u[me] = r_u;
+  s->rhs_v[me] = r_v;
+}
+
+/* calculate_RHS_mod.f90:36-50: classical SPH for SPH_MODEL <= 2, LS type A
+   otherwise. */
 int32_t orc_calculate_rhs(orc_sim *s) {
   if (!s->cells_ready) { set_err(s, "calculate_rhs before init_cell"); return LSSPH_E_STATE; }
+  const int classical = s->p.sph_model <= 2;
 #pragma omp parallel for schedule(static)
-  for (int me = 0; me < s->n_int; ++me) lssph_a(s, me);
+  for (int me = 0; me < s->n_int; ++me) {
+    if (classical)
+      classical_sph(s, me);
+    else
+      lssph_a(s, me);
+  }
   return 0;
 }
 

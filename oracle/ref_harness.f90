@@ -26,7 +26,9 @@ program ref_harness
     use calculate_RHS_mod       , only: calculate_RHS
     use RK2_mod                 , only: RK2
     use RK4_mod                 , only: RK4
+#if (TARGET_PROBLEM != 1)
     use PST_mod                 , only: PST
+#endif
     use check_steady_state_mod  , only: check_steady_state
     implicit none
 

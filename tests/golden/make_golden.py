@@ -42,6 +42,13 @@ CASES = {
     "tg_quintic_rect": (0.1, 107, 10, None),
     "tg_wc6_rk4": (0.2, 108, 6, None),
     "tg_wc4_rk2": (0.15, 109, 6, None),
+    "tg_sph1_rk2": (0.2, 110, 6, None),
+    "tg_sph2_rk4_delta": (0.2, 111, 6, None),
+    "diff_sph2_rk2": (0.1, 112, 6, "smooth"),
+    "bous_sph1_rk2": (0.2, 113, 6, None),
+    "tg_ls9_rk2": (0.2, 114, 6, None),
+    "diff_ls14_rk4": (0.1, 115, 6, "smooth"),
+    "tg_ls20_rk2": (0.1, 116, 4, None),
 }
 
 

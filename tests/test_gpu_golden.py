@@ -7,10 +7,11 @@ must be bit-exact (same sets, same summation order).
 import numpy as np
 import pytest
 
-from helpers import CASES, Golden, max_rel, rel_err
+from helpers import CASES as ALL_CASES, Golden, case_tol, max_rel, rel_err
 
 pytestmark = pytest.mark.gpu
-TOL = 1e-12
+# cases whose operators the HIP library runs
+CASES = [c for c in ALL_CASES if Golden(c).cfg.sph_model == 3]
 
 
 def _sim(g: Golden, st=None, precision=0):
@@ -26,11 +27,11 @@ def test_initial_ghost_markers_cells(case):
         assert np.array_equal(s.cell_index(), g["cell0"])
         s.ghost()
         errs = max_rel(s.download(), g, "state0")
-        assert max(errs.values()) <= TOL, errs
+        assert max(errs.values()) <= case_tol(case), errs
         vm = s.download_vm()
         assert np.array_equal(vm["ptype"], g["vm_ptype"])
         for k in ("x", "y", "u", "v", "pre") + (("tem",) if g.has_tem else ()):
-            assert rel_err(vm[k], g[f"vm_{k}"]) <= TOL, k
+            assert rel_err(vm[k], g[f"vm_{k}"]) <= case_tol(case), k
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -50,7 +51,7 @@ def test_first_rhs(case):
         r = s.calculate_RHS()
         for k in r:
             if g.has(f"rhs0_{k}"):
-                assert rel_err(r[k], g[f"rhs0_{k}"]) <= TOL, k
+                assert rel_err(r[k], g[f"rhs0_{k}"]) <= case_tol(case), k
 
 
 @pytest.mark.parametrize("case", [c for c in CASES if not c.startswith("diff")])
@@ -60,7 +61,7 @@ def test_one_pst_pass(case):
         s.ghost()
         s.PST()
         errs = max_rel(s.download(), g, "pst0", fields=("x", "y", "u", "v", "rho", "pre"))
-        assert max(errs.values()) <= TOL, errs
+        assert max(errs.values()) <= case_tol(case), errs
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -70,10 +71,10 @@ def test_trajectory_eager(case):
         s.ghost()
         s.eager_step(1)
         e1 = max_rel(s.download(), g, "step_1")
-        assert max(e1.values()) <= TOL, e1
+        assert max(e1.values()) <= case_tol(case), e1
         s.eager_step(g.steps - 1)
         el = max_rel(s.download(), g, "step_last")
-        assert max(el.values()) <= TOL, el
+        assert max(el.values()) <= case_tol(case), el
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -88,7 +89,7 @@ def test_trajectory_graph_matches_eager_bitwise(case):
         for k in g.fields():
             assert np.array_equal(da[k], db[k]), k
         el = max_rel(db, g, "step_last")
-        assert max(el.values()) <= TOL, el
+        assert max(el.values()) <= case_tol(case), el
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -97,7 +98,7 @@ def test_per_step_from_reference_state(case):
     with _sim(g, g.state("step_1")) as s:
         s.eager_step(1)
         e = max_rel(s.download(), g, "step_2")
-        assert max(e.values()) <= TOL, e
+        assert max(e.values()) <= case_tol(case), e
 
 
 def test_diffusion_steady_flags():

@@ -190,11 +190,14 @@ def _exact_rhs(param, x, y):
     return out
 
 
-@pytest.mark.parametrize("case", ["tg_rk4_delta", "cavity_rk2", "bous_rk4_wm3", "diff_rk4", "tg_quintic_rect",
-                                  "tg_wc6_rk4"])
-def test_ls_type_a_exact_for_quadratic_fields(case):
+@pytest.mark.parametrize("case,tol", [("tg_rk4_delta", 1e-10), ("cavity_rk2", 1e-10), ("bous_rk4_wm3", 1e-10),
+                                      ("diff_rk4", 1e-10), ("tg_quintic_rect", 1e-10), ("tg_wc6_rk4", 1e-10),
+                                      ("tg_ls9_rk2", 1e-10), ("diff_ls14_rk4", 1e-10), ("tg_ls20_rk2", 1e-7)])
+def test_ls_type_a_exact_for_quadratic_fields(case, tol):
     """Every fluid particle, walls included as neighbors (their values set to
-    the same quadratic, no ghost fill): the RHS is the closed form."""
+    the same quadratic, no ghost fill): the RHS is the closed form, for every
+    Taylor order q = 5, 9, 14, 20 (a quadratic lies in each basis).  The
+    q = 20 bound is looser: cond(M_20) reaches 4e7 on this lattice."""
     g = Golden(case)
     st = _quadratic_state(g.state("state0"))
     o = OracleSim(g.param.to_c(), g.n_int, g.n_ext)
@@ -203,7 +206,88 @@ def test_ls_type_a_exact_for_quadratic_fields(case):
     n = g.n_int
     exact = _exact_rhs(g.param, st.x[:n], st.y[:n])
     for k, ref in exact.items():
-        assert rel_err(got[k], ref) <= 1e-10, k
+        assert rel_err(got[k], ref) <= tol, k
+
+
+class Quartic:
+    """A quartic polynomial in x and y (every monomial of degree <= 4)."""
+
+    def __init__(self, seed):
+        self.c = np.random.default_rng(seed).uniform(-1.0, 1.0, 15)
+
+    def __call__(self, x, y):
+        out, k = 0.0, 0
+        for d in range(5):
+            for j in range(d + 1):
+                out = out + self.c[k] * x ** (d - j) * y ** j
+                k += 1
+        return out
+
+
+@pytest.mark.parametrize("case,degree,tol", [("tg_sph1_rk2", 1, 1e-13), ("tg_rk2", 2, 1e-13),
+                                             ("tg_ls9_rk2", 3, 1e-13), ("diff_ls14_rk4", 4, 1e-13),
+                                             ("tg_ls20_rk2", 4, 1e-11)])
+def test_pst_interpolation_exact_for_polynomial_fields(case, degree, tol):
+    """One shifting pass moves each fluid particle by dr and re-interpolates
+    u, v, rho with the Taylor basis of its model (shift_interpolate_mod.f90:
+    255-298): a field of degree <= k (k = 1, 2, 3, 4, 5 for q = 2, 5, 9, 14,
+    20) comes out as the field at the new position.  This pins the monomial
+    set of the q = 14 basis (SPH_MODEL 5), whose reference shifting does not
+    compile, and the q = 2 fit of the classical models; the shifts reach the
+    0.2 h clamp, so degree-4 terms weigh ~1e-7 here.  (A constant factor on
+    one basis entry leaves the span, and so every result, unchanged.)"""
+    g = Golden(case)
+    st = g.state("state0").copy()
+    f = [Quartic(s) for s in (1, 2, 3)]
+    if degree < 4:  # drop the monomials of degree > `degree`
+        for q in f:
+            q.c[(degree + 1) * (degree + 2) // 2:] = 0.0
+    f[2].c *= 0.01
+    f[2].c[0] = 1.0  # rho = 1 + O(0.01)
+    st.u, st.v, st.rho = f[0](st.x, st.y), f[1](st.x, st.y), f[2](st.x, st.y)
+    st.u = st.u + 500.0  # U_max large enough that shifts reach the 0.2 h clamp
+    o = OracleSim(g.param.to_c(), g.n_int, g.n_ext)
+    o.init(st)
+    o.pst()
+    got = o.get_state()
+    n = g.n_int
+    moved = np.hypot(got["x"][:n] - st.x[:n], got["y"][:n] - st.y[:n])
+    assert moved.max() > 0.19 * g.param.h
+    x, y = got["x"][:n], got["y"][:n]
+    assert rel_err(got["u"][:n], f[0](x, y) + 500.0) <= tol
+    assert rel_err(got["v"][:n], f[1](x, y)) <= tol
+    assert rel_err(got["rho"][:n], f[2](x, y)) <= tol
+
+
+def test_classical_sph_gradient_of_constant_pressure():
+    """Classical SPH (classical_SPH_mod.f90:96-136) on a fluid at rest with
+    uniform density and pressure p0: the difference model (SPH_MODEL 2) gives
+    exactly zero for every RHS; the sum model (SPH_MODEL 1) gives
+    RHS_u = -2 p0 sum_j dW_x V_j / rho, i.e. -2 p0 / rho times the kernel
+    gradient sum, which vanishes by symmetry at the centre of a regular
+    lattice."""
+    g = Golden("tg_sph1_rk2")
+    st = g.state("setup").copy()
+    n = g.n_int
+    st.u[:] = 0.0
+    st.v[:] = 0.0
+    st.rho[:] = 1.0
+    st.pre[:] = 0.7
+    for model in (1, 2):
+        cfg = g.cfg.replace(sph_model=model)
+        param = lcfg.init_param(cfg)
+        o = OracleSim(param.to_c(), n, g.n_ext)
+        o.init(st)
+        r = o.calculate_rhs()
+        if model == 2:
+            for k in ("u", "v", "rho"):
+                assert np.all(r[k] == 0.0), k
+        else:
+            # interior particles of the (unjittered) lattice: symmetric stencil
+            m = _interior(st.x[:n] - param.wl_thick, st.y[:n] - param.wl_thick, 1.05 * param.h_eff)
+            assert m.sum() >= 4
+            assert np.max(np.abs(r["u"][m])) <= 1e-9 * 0.7 / param.dx
+            assert np.max(np.abs(r["v"][m])) <= 1e-9 * 0.7 / param.dx
 
 
 # --- periodic box (extension) -----------------------------------------------------------------
