@@ -73,7 +73,10 @@ typedef struct lssph_params {
   /* model options (config.h macros) */
   int32_t target_problem;   /* TARGET_PROBLEM 1..4                            */
   int32_t kernel_type;      /* KERNEL_TYPE 1..5                               */
-  int32_t sph_model;        /* SPH_MODEL: 3 (LS-SPH, 2nd-order Taylor)        */
+  int32_t sph_model;        /* SPH_MODEL 1..6: 1-2 classical SPH (sum /
+                               difference pressure gradient), 3-6 LS-SPH type
+                               A with q = 5, 9, 14, 20; LSSPH_MIXED applies to
+                               3, the other models compute in FP64           */
   int32_t wall_model;       /* WALL_MODEL 1..3                                */
   int32_t rk;               /* RK 2 or 4                                      */
   int32_t u_bc_top, u_bc_bottom, u_bc_left, u_bc_right;       /* U_BOUNDARY_*   */

@@ -97,7 +97,7 @@ int validate(const lssph_params_t *p, int n_inst, const int32_t *n_int, const in
   const lssph_params_t &a = p[0];
   if (a.target_problem < 1 || a.target_problem > 4) return msg = "target_problem must be 1..4", 1;
   if (a.kernel_type < 1 || a.kernel_type > 5) return msg = "kernel_type must be 1..5", 1;
-  if (a.sph_model != 3) return msg = "sph_model must be 3 (LS-SPH, 2nd-order Taylor)", 1;
+  if (a.sph_model < 1 || a.sph_model > 6) return msg = "sph_model must be 1..6", 1;
   if (a.wall_model < 1 || a.wall_model > 3) return msg = "wall_model must be 1..3", 1;
   if (a.rk != 2 && a.rk != 4) return msg = "rk must be 2 or 4", 1;
   if (a.precision != LSSPH_FP64 && a.precision != LSSPH_MIXED) return msg = "precision must be 0 or 1", 1;
@@ -213,6 +213,13 @@ int check_counters(lssph_handle *h) {
     return fail(h, LSSPH_E_DOMAIN,
                 "a particle moved farther than half the cell-list skin between cell builds (lssph_set_skin "
                 "raises the skin; 0 rebuilds the cells after every pass)");
+  }
+  if (c[kWaveListOverflow] > 0) {
+    char buf[200];
+    std::snprintf(buf, sizeof buf,
+                  "%d particle(s) have more than %d neighbors, the limit of the SPH_MODEL 4-6 kernels",
+                  c[kWaveListOverflow], kWaveList);
+    return fail(h, LSSPH_E_CAPACITY, buf);
   }
   if (c[kStrayOverflow] > 0) {
     char buf[200];
@@ -445,7 +452,7 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   const lssph_params_t &a = params[0];
   d.opt = Options{a.target_problem, a.kernel_type, a.wall_model, a.rk, a.u_bc_top, a.u_bc_bottom,
                   a.u_bc_left, a.u_bc_right, a.tem_bc_top, a.tem_bc_bottom, a.tem_bc_left, a.tem_bc_right,
-                  a.target_problem == 4 ? 1 : 0, a.precision, 0, 0};
+                  a.target_problem == 4 ? 1 : 0, a.precision, 0, a.sph_model};
   for (int k = 0; k < n_inst; ++k)
     if (params[k].coe_delta_sph != 0.0) d.opt.delta = 1;
   {

@@ -57,8 +57,8 @@ struct Options {
   int32_t u_bc_top, u_bc_bottom, u_bc_left, u_bc_right;
   int32_t tem_bc_top, tem_bc_bottom, tem_bc_left, tem_bc_right;
   int32_t has_tem, precision;
-  int32_t delta;  // any instance with coe_delta_sph != 0
-  int32_t pad;
+  int32_t delta;      // any instance with coe_delta_sph != 0
+  int32_t sph_model;  // 1-2 classical SPH, 3-6 LS type A with q = 5, 9, 14, 20
 };
 
 // Hardware square root and reciprocal of FP32 (1 ulp), for the mixed path's

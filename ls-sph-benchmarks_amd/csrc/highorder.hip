@@ -1,0 +1,588 @@
+// highorder.hip -- LS type A with the 3rd- to 5th-order Taylor bases
+// (SPH_MODEL 4-6: q = 9, 14, 20; LSSPH_A_mod.f90:46-53, 148-172) and the
+// particle shifting with the same bases (shift_interpolate_mod.f90:45-52,
+// 153-185, 255-298), FP64.
+//
+// A q x q moment matrix (q = 20: 210 sums plus 20 per field) does not fit
+// one thread's registers, so one wavefront handles one particle:
+//   1. the lanes test the stencil's candidates 64 at a time and compact the
+//      accepted ones into an LDS list with a ballot;
+//   2. a rank sort puts the list into the reference's summation order
+//      (position of the neighbor's reference cell in the 3x3 stencil of the
+//      particle's reference cell, then upload index): at q = 20, whose
+//      matrices reach cond(M) ~ 4e7, the summation order shows in the results;
+//   3. in chunks of 32 neighbors, lane k forms the basis a, VW * a and the
+//      field differences of neighbor k in LDS; then every lane accumulates
+//      its share of the sums over the chunk, in list order;
+//   4. the wave factors the symmetrized matrix in LDS (LU with partial
+//      pivoting, the operation order of lu_factor / lu_apply) and solves;
+//   5. lane 0 runs the epilogue of the q = 5 kernels (RHS, stage update,
+//      cell bookkeeping).
+// Waves never wait for each other (no block barrier), so a wave whose slot
+// is a wall or past the end simply returns.
+#include "ls_common.hpp"
+
+namespace lssph {
+
+constexpr int kChunk = 32;   // neighbors staged in LDS per round
+constexpr int kWavesPB = 2;  // wavefronts (particles) per block
+
+// Orders the LDS accesses of the lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Kernel W(r) and dW/dr written term by term as kernel_functions_mod.f90
+// forms them (q = r / h, powers as repeated products, normalisation applied
+// last): with cond(M_20) up to 4e7, a last-bit difference in W shows.
+__device__ __forceinline__ double ipow_d(double t, int n) {
+  double r = t;
+  for (int i = 1; i < n; ++i) r *= t;
+  return r;
+}
+
+__device__ double ref_w(int kt, double r, double h) {
+  const double q = r / h;
+  const double pi = kPi;
+  double w;
+  switch (kt) {
+    case 1:  // :33-41
+      w = (0.0 <= q && q <= 1.0) ? ipow_d(2.0 - q, 3) - 4.0 * ipow_d(1.0 - q, 3)
+                                 : ((1.0 < q && q <= 2.0) ? ipow_d(2.0 - q, 3) : 0.0);
+      return 5.0 / (14.0 * pi * (h * h)) * w;
+    case 2:  // :43-53
+      if (0.0 <= q && q <= 1.0)
+        w = ipow_d(3.0 - q, 5) - 6.0 * ipow_d(2.0 - q, 5) + 15.0 * ipow_d(1.0 - q, 5);
+      else if (1.0 < q && q <= 2.0)
+        w = ipow_d(3.0 - q, 5) - 6.0 * ipow_d(2.0 - q, 5);
+      else
+        w = (2.0 < q && q <= 3.0) ? ipow_d(3.0 - q, 5) : 0.0;
+      return 7.0 / (478.0 * pi * (h * h)) * w;
+    case 3:  // :55-61
+      w = (0.0 <= q && q <= 2.0) ? ipow_d(1.0 - 0.5 * q, 4) * (1.0 + 2.0 * q) : 0.0;
+      return 7.0 / (4.0 * pi * (h * h)) * w;
+    case 4:  // :63-69
+      w = (0.0 <= q && q <= 2.0) ? ipow_d(1.0 - 0.5 * q, 6) * (1.0 + 3.0 * q + 35.0 * (q * q) / 12.0) : 0.0;
+      return 9.0 / (4.0 * pi * (h * h)) * w;
+    default:  // :71-78
+      w = (0.0 <= q && q <= 2.0)
+              ? ipow_d(1.0 - 0.5 * q, 8) * (1.0 + 4.0 * q + 25.0 * (q * q) / 4.0 + 4.0 * (q * q * q))
+              : 0.0;
+      return 39.0 / (14.0 * pi * (h * h)) * w;
+  }
+}
+
+__device__ double ref_dw(int kt, double r, double h) {
+  const double q = r / h;
+  const double pi = kPi;
+  double d;
+  switch (kt) {
+    case 1:  // :94-102
+      d = (0.0 <= q && q <= 1.0) ? ipow_d(2.0 - q, 2) - 4.0 * ipow_d(1.0 - q, 2)
+                                 : ((1.0 < q && q <= 2.0) ? ipow_d(2.0 - q, 2) : 0.0);
+      return 5.0 / (14.0 * pi * (h * h)) * d * (-3.0 / h);
+    case 2:  // :104-114
+      if (0.0 <= q && q <= 1.0)
+        d = ipow_d(3.0 - q, 4) - 6.0 * ipow_d(2.0 - q, 4) + 15.0 * ipow_d(1.0 - q, 4);
+      else if (1.0 < q && q <= 2.0)
+        d = ipow_d(3.0 - q, 4) - 6.0 * ipow_d(2.0 - q, 4);
+      else
+        d = (2.0 < q && q <= 3.0) ? ipow_d(3.0 - q, 4) : 0.0;
+      return 7.0 / (478.0 * pi * (h * h)) * d * (-5.0 / h);
+    case 3:  // :116-123
+      d = (0.0 <= q && q <= 2.0)
+              ? -2.0 * ipow_d(1.0 - 0.5 * q, 3) * (1.0 + 2.0 * q) + 2.0 * ipow_d(1.0 - 0.5 * q, 4)
+              : 0.0;
+      return 7.0 / (4.0 * pi * (h * h)) * d / h;
+    case 4:  // :125-132
+      d = (0.0 <= q && q <= 2.0) ? -3.0 * ipow_d(1.0 - 0.5 * q, 5) * (1.0 + 3.0 * q + 35.0 * (q * q) / 12.0) +
+                                       1.0 * ipow_d(1.0 - 0.5 * q, 6) * (3.0 + 35.0 * q / 6.0)
+                                 : 0.0;
+      return 9.0 / (4.0 * pi * (h * h)) * d / h;
+    default:  // :134-143
+      d = (0.0 <= q && q <= 2.0)
+              ? -4.0 * ipow_d(1.0 - 0.5 * q, 7) * (1.0 + 4.0 * q + 25.0 * (q * q) / 4.0 + 4.0 * (q * q * q)) +
+                    1.0 * ipow_d(1.0 - 0.5 * q, 8) * (4.0 + 25.0 * q / 2.0 + 12.0 * (q * q))
+              : 0.0;
+      return 39.0 / (14.0 * pi * (h * h)) * d / h;
+  }
+}
+
+// Taylor basis of q terms with ih[k] = 1/h^(k+1) (LSSPH_A_mod.f90:140-170,
+// shift_interpolate_mod.f90:153-185 and 259-297).
+template <int Q>
+__device__ __forceinline__ void basis_hq(double x, double y, const double (&ih)[5], double (&a)[Q]) {
+  a[0] = x * ih[0];
+  a[1] = y * ih[0];
+  a[2] = 0.5 * (x * x) * ih[1];
+  a[3] = 0.5 * (y * y) * ih[1];
+  a[4] = x * y * ih[1];
+  a[5] = 1.0 / 6.0 * (x * x * x) * ih[2];
+  a[6] = 1.0 / 6.0 * (y * y * y) * ih[2];
+  a[7] = 0.5 * (x * x) * y * ih[2];
+  a[8] = 0.5 * (y * y) * x * ih[2];
+  if constexpr (Q >= 14) {
+    a[9] = 1.0 / 24.0 * (x * x * x * x) * ih[3];
+    a[10] = 1.0 / 24.0 * (y * y * y * y) * ih[3];
+    a[11] = 1.0 / 6.0 * (x * x * x) * y * ih[3];
+    a[12] = 1.0 / 6.0 * (y * y * y) * x * ih[3];
+    a[13] = 1.0 / 4.0 * (x * x) * (y * y) * ih[3];
+  }
+  if constexpr (Q >= 20) {
+    a[14] = 1.0 / 120.0 * (x * x * x * x * x) * ih[4];
+    a[15] = 1.0 / 120.0 * (y * y * y * y * y) * ih[4];
+    a[16] = 1.0 / 24.0 * (x * x * x * x) * y * ih[4];
+    a[17] = 1.0 / 24.0 * (y * y * y * y) * x * ih[4];
+    a[18] = 1.0 / 12.0 * (x * x * x) * (y * y) * ih[4];
+    a[19] = 1.0 / 12.0 * (y * y * y) * (x * x) * ih[4];
+  }
+}
+
+// LDS of one wavefront.  Row k of `rows` holds neighbor k of the chunk:
+// a[0..Q), VW * a[0..Q), the field differences [0..NF) and the two shift
+// terms of the shifting kernel.  The factorization reuses the same space.
+template <int Q, int NF>
+struct WaveLds {
+  static constexpr int RS = 2 * Q + NF + 2;
+  int32_t list[kWaveList];    // accepted candidates, stencil order
+  int32_t sorted[kWaveList];  // the same in summation order
+  long long key[kWaveList];
+  union {
+    double rows[kChunk][RS];
+    struct {
+      double m[Q][Q];
+      double b[Q][NF];
+    } s;
+  } u;
+};
+
+// Neighbors of the particle at slot s (record me) within h_eff, in the
+// reference's summation order, into W.sorted.  Returns the count; a count
+// above kWaveList is flagged and only the first kWaveList are kept.
+template <class WL>
+__device__ int wave_neighbors(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
+                              const int32_t *__restrict__ sinfo, const int32_t *__restrict__ items,
+                              const int32_t *__restrict__ orig_of, int32_t *counters, const InstDev &I, int s,
+                              const Rec &me, WL &W, int lane) {
+  const int lc = local_cell(I, me.x, me.y);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int cnt = 0;
+  visit_cells(I, lc, cstart, [&](int t0, int t1, int, int) {
+    for (int base = t0; base < t1; base += 64) {
+      const int t = base + lane;
+      bool ok = false;
+      int code = 0;
+      if (t < t1 && t != s) {
+        const double2 p = load_xy(mir, t);
+        const double sx = I.periodic ? image_shift(p.x, me.x, I.len_x) : 0.0;
+        const double sy = I.periodic ? image_shift(p.y, me.y, I.len_y) : 0.0;
+        ok = sq_dist((p.x + sx) - me.x, (p.y + sy) - me.y) <= I.h_eff_sq;
+        code = image_code(sx, sy);
+      }
+      const unsigned long long m = __ballot(ok);
+      const int pos = cnt + __popcll(m & below);
+      if (ok && pos < kWaveList) W.list[pos] = t | (code << kSlotBits);
+      cnt += __popcll(m);
+    }
+  });
+  if (cnt > kWaveList) {
+    if (lane == 0) atomicAdd(&counters[kWaveListOverflow], 1);
+    cnt = kWaveList;
+  }
+  // sort keys: stencil position of the reference cells, then upload index
+  int rcx, rcy;
+  ref_cell_xy(I, me.x, me.y, false, rcx, rcy);
+  for (int e = lane; e < cnt; e += 64) {
+    const int t = W.list[e] & kSlotMask;
+    const double2 p = load_xy(mir, t);
+    int jx, jy;
+    ref_cell_xy(I, p.x, p.y, info_ptype(sinfo[t]) != 0, jx, jy);
+    int dx = jx - rcx, dy = jy - rcy;
+    if (I.periodic) {
+      dx = dx > 1 ? dx - I.rnx : (dx < -1 ? dx + I.rnx : dx);
+      dy = dy > 1 ? dy - I.rny : (dy < -1 ? dy + I.rny : dy);
+    }
+    W.key[e] = ((long long)((dy + 1) * 3 + (dx + 1)) << 32) | (long long)orig_of[items[t]];
+  }
+  wave_sync();
+  // rank sort (the keys are distinct)
+  for (int e = lane; e < cnt; e += 64) {
+    const long long k = W.key[e];
+    int rank = 0;
+    for (int j = 0; j < cnt; ++j) rank += (W.key[j] < k) ? 1 : 0;
+    W.sorted[rank] = W.list[e];
+  }
+  wave_sync();
+  return cnt;
+}
+
+// Per-lane share of the LS sums: entry e < Q(Q+1)/2 is M(r, i), r <= i,
+// accumulated as (VW a_i) * a_r (LSSPH_A_mod.f90:185-188); the others are
+// b(i, f) += (VW a_i) * df_f (:178-183).  o1/o2 are the row offsets of the
+// two factors.
+template <int Q, int NF>
+struct LaneSums {
+  static constexpr int NM = Q * (Q + 1) / 2;
+  static constexpr int NE = NM + Q * NF;
+  static constexpr int PER = (NE + 63) / 64;
+  double acc[PER];
+  int o1[PER], o2[PER];
+  __device__ __forceinline__ void init(int lane) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      acc[m] = 0.0;
+      const int e = lane + 64 * m;
+      int i = 0, r = 0, f = -1;
+      if (e < NM) {
+        while ((i + 1) * (i + 2) / 2 <= e) ++i;
+        r = e - i * (i + 1) / 2;
+      } else if (e < NE) {
+        f = (e - NM) / Q;
+        i = (e - NM) % Q;
+      }
+      o1[m] = Q + i;
+      o2[m] = f < 0 ? r : 2 * Q + f;
+    }
+  }
+  template <class WL>
+  __device__ __forceinline__ void add_chunk(const WL &W, int nc, int lane) {
+    for (int k = 0; k < nc; ++k) {
+      const double *row = W.u.rows[k];
+#pragma unroll
+      for (int m = 0; m < PER; ++m)
+        if (lane + 64 * m < NE) acc[m] += row[o1[m]] * row[o2[m]];
+    }
+  }
+  // symmetrized matrix and right-hand sides into W.u.s (after a wave_sync)
+  template <class WL>
+  __device__ __forceinline__ void store(WL &W, int lane) const {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int e = lane + 64 * m;
+      if (e < NM) {
+        const int i = o1[m] - Q, r = o2[m];
+        W.u.s.m[r][i] = acc[m];
+        W.u.s.m[i][r] = acc[m];
+      } else if (e < NE) {
+        W.u.s.b[o1[m] - Q][o2[m] - 2 * Q] = acc[m];
+      }
+    }
+  }
+};
+
+// DGESV on W.u.s by one wavefront: LU with partial pivoting in the order
+// of lu_factor (first row of the largest |a| as pivot; a zero pivot sets
+// info and skips the scaling), then -- when no pivot is zero -- the row
+// interchanges, forward and back substitution of lu_apply.  Returns info.
+template <int Q, int NF, class WL>
+__device__ int wave_dgesv(WL &W, int lane) {
+  auto &m = W.u.s.m;
+  auto &b = W.u.s.b;
+  int piv[Q];
+  int info = 0;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    double v = (lane >= j && lane < Q) ? fabs(m[lane < Q ? lane : 0][j]) : -1.0;
+    int idx = lane;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double v2 = __shfl_xor(v, off);
+      const int i2 = __shfl_xor(idx, off);
+      const bool take = (v2 > v) || (v2 == v && i2 < idx);
+      v = take ? v2 : v;
+      idx = take ? i2 : idx;
+    }
+    int p = __builtin_amdgcn_readfirstlane(idx);
+    p = (p >= j && p < Q) ? p : j;  // only a column of NaNs leaves no pivot row
+    piv[j] = p;
+    if (p != j && lane < Q) {
+      const double t = m[j][lane];
+      m[j][lane] = m[p][lane];
+      m[p][lane] = t;
+    }
+    wave_sync();
+    const double pv = m[j][j];
+    if (pv == 0.0) {
+      info = info ? info : j + 1;
+    } else if (lane > j && lane < Q) {
+      m[lane][j] = m[lane][j] / pv;
+    }
+    wave_sync();
+    const int nn = Q - 1 - j;
+    for (int e = lane; e < nn * nn; e += 64) {
+      const int i = j + 1 + e / nn, c = j + 1 + e % nn;
+      m[i][c] -= m[i][j] * m[j][c];
+    }
+    wave_sync();
+  }
+  if (info) return info;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const int p = piv[j];
+    if (p != j && lane < NF) {
+      const double t = b[j][lane];
+      b[j][lane] = b[p][lane];
+      b[p][lane] = t;
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const int rows = (Q - 1 - j) * NF;
+    for (int e = lane; e < rows; e += 64) {
+      const int i = j + 1 + e / NF, f = e % NF;
+      b[i][f] -= b[j][f] * m[i][j];
+    }
+    wave_sync();
+  }
+#pragma unroll
+  for (int j = Q - 1; j >= 0; --j) {
+    if (lane < NF) b[j][lane] = b[j][lane] / m[j][j];
+    wave_sync();
+    for (int e = lane; e < j * NF; e += 64) {
+      const int i = e / NF, f = e % NF;
+      b[i][f] -= b[j][f] * m[i][j];
+    }
+    wave_sync();
+  }
+  return 0;
+}
+
+__device__ __forceinline__ void inv_h_powers(const InstDev &I, double (&ih)[5]) {
+  ih[0] = I.inv_h;
+  ih[1] = I.inv_h2;  // (1/h)/h as the reference's inv_h(2)
+#pragma unroll
+  for (int k = 2; k < 5; ++k) ih[k] = ih[k - 1] / I.h;
+}
+
+// ---------------------------------------------------------------------------
+// LS type A (LSSPH_A_mod.f90:26-271) with q = 9, 14, 20 and the stage update.
+// The field columns are those of the q = 5 kernels with rho always carried.
+// ---------------------------------------------------------------------------
+template <int Q, int CLS>
+__global__ void __launch_bounds__(64 * kWavesPB) k_lsa_wave(LsaArgs A) {
+  using L = Layout<CLS, true>;
+  constexpr int NF = L::NF;
+  using WL = WaveLds<Q, NF>;
+  __shared__ WL lds_all[kWavesPB];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int s = blockIdx.x * kWavesPB + w;
+  if (s >= A.n) return;
+  WL &W = lds_all[w];
+  const int32_t inf = A.sinfo[s];
+  if (info_ptype(inf) != 0) {
+    if (lane == 0 && A.mode != kRhsOnly) A.mout[s] = A.mir[s];
+    return;
+  }
+  const InstDev &I = A.inst[info_inst(inf)];
+  const Rec me = A.mir[s];
+  const int n = wave_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.counters, I, s, me, W, lane);
+  double ih[5];
+  inv_h_powers(I, ih);
+  const int kt = A.kt;
+  LaneSums<Q, NF> S;
+  S.init(lane);
+  for (int c0 = 0; c0 < n; c0 += kChunk) {
+    const int nc = min(kChunk, n - c0);
+    if (lane < nc) {
+      const int e = W.sorted[c0 + lane];
+      const int t = e & kSlotMask, code = e >> kSlotBits;
+      const Rec nb = A.mir[t];
+      const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
+      const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
+      const double r = sqrt(sq_dist(dx, dy));
+      const double vw = nb.vol * ref_w(kt, r, I.h);
+      double a[Q];
+      basis_hq<Q>(dx, dy, ih, a);
+      double df[NF];
+      field_diffs<CLS, true>(nb, me, df);
+      double *row = W.u.rows[lane];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        row[i] = a[i];
+        row[Q + i] = vw * a[i];
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) row[2 * Q + f] = df[f];
+    }
+    wave_sync();
+    S.add_chunk(W, nc, lane);
+    wave_sync();
+  }
+  S.store(W, lane);
+  wave_sync();
+  double d[NF][5];
+  if (n >= Q) {  // LSSPH_A_mod.f90:208 (num_neighbors >= q)
+    wave_dgesv<Q, NF>(W, lane);
+    double X[NF][5];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int i = 0; i < 5; ++i) X[f][i] = W.u.s.b[i][f];
+    scale_derivs<NF>(X, I, d);
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int i = 0; i < 5; ++i) d[f][i] = 0.0;
+  }
+  if (lane == 0) finish<CLS, true>(A, I, A.items[s], s, me, d);
+}
+
+// ---------------------------------------------------------------------------
+// Particle shifting (shift_interpolate_mod.f90:23-300) with q = 9, 14, 20.
+// ---------------------------------------------------------------------------
+template <int Q, bool T>
+__global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
+  constexpr int NF = T ? 4 : 3;  // u, v, rho (, tem)
+  using WL = WaveLds<Q, NF>;
+  __shared__ WL lds_all[kWavesPB];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int s = blockIdx.x * kWavesPB + w;
+  if (s >= A.n) return;
+  WL &W = lds_all[w];
+  const int32_t inf = A.sinfo[s];
+  if (info_ptype(inf) != 0) {
+    if (lane == 0) A.mout[s] = A.mir[s];  // walls move on with the records
+    return;
+  }
+  const int ii = info_inst(inf);
+  const InstDev &I = A.inst[ii];
+  const Rec me = A.mir[s];
+  const int n = wave_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.mv.counters, I, s, me, W, lane);
+  double ih[5];
+  inv_h_powers(I, ih);
+  const int kt = A.kt;
+  LaneSums<Q, NF> S;
+  S.init(lane);
+  double sx = 0.0, sy = 0.0;
+  for (int c0 = 0; c0 < n; c0 += kChunk) {
+    const int nc = min(kChunk, n - c0);
+    if (lane < nc) {
+      const int e = W.sorted[c0 + lane];
+      const int t = e & kSlotMask, code = e >> kSlotBits;
+      const Rec nb = A.mir[t];
+      const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
+      const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
+      const double r = sqrt(sq_dist(dx, dy));
+      const double wv = ref_w(kt, r, I.h);
+      const double dwdr = ref_dw(kt, r, I.h);
+      const double vj = nb.vol;
+      const double vw = vj * wv;
+      const double dwx = dwdr * (-dx) / r * vj;  // :142-143
+      const double dwy = dwdr * (-dy) / r * vj;
+      const double wr = wv / I.w_ave;
+      const double fac = 1.0 + 0.2 * ((wr * wr) * (wr * wr));
+      double a[Q];
+      basis_hq<Q>(dx, dy, ih, a);
+      double *row = W.u.rows[lane];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        row[i] = a[i];
+        row[Q + i] = vw * a[i];
+      }
+      row[2 * Q + 0] = nb.u - me.u;
+      row[2 * Q + 1] = nb.v - me.v;
+      row[2 * Q + 2] = nb.rho - me.rho;
+      if constexpr (T) row[2 * Q + 3] = nb.tem - me.tem;
+      row[2 * Q + NF] = fac * dwx;
+      row[2 * Q + NF + 1] = fac * dwy;
+    }
+    wave_sync();
+    for (int k = 0; k < nc; ++k) {  // the shift sums in list order (:145-146)
+      sx = sx + W.u.rows[k][2 * Q + NF];
+      sy = sy + W.u.rows[k][2 * Q + NF + 1];
+    }
+    S.add_chunk(W, nc, lane);
+    wave_sync();
+  }
+  S.store(W, lane);
+  wave_sync();
+  // shift vector and its clamp (:217-225)
+  const double umax = __longlong_as_double((long long)A.umax[ii]);
+  const double coe = I.pst_c * umax * I.dt * I.h;
+  double dxs = -coe * sx, dys = -coe * sy;
+  const double rs = sqrt(dxs * dxs + dys * dys);
+  if (rs > 0.2 * I.h) {
+    dxs = (0.2 * I.h) * dxs / rs;
+    dys = (0.2 * I.h) * dys / rs;
+  }
+  bool solved = false;
+  if (n >= Q) {  // :240-247
+    wave_dgesv<Q, NF>(W, lane);
+    solved = true;
+  }
+  if (lane != 0) return;
+  double a[Q];
+  basis_hq<Q>(dxs, dys, ih, a);
+  double acc[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    double t = 0.0;
+    if (solved) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) t += a[i] * W.u.s.b[i][f];
+    }
+    acc[f] = t;
+  }
+  Rec out = me;
+  const bool per = I.periodic;
+  out.x = wrap_pos(me.x + dxs, I.len_x, per);
+  out.y = wrap_pos(me.y + dys, I.len_y, per);
+  out.u = me.u + acc[0];
+  out.v = me.v + acc[1];
+  out.rho = me.rho + acc[2];
+  if constexpr (T) out.tem = me.tem + acc[3];
+  out.pre = I.c_sq * (out.rho - I.rho_ref);
+  out.vol = I.sp_mass / out.rho;
+  A.mout[s] = out;
+  record_move(A.mv, I, s, out.x, out.y);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <int Q>
+static void lsa_wave_q(hipStream_t s, const LsaArgs &a, int cls) {
+  const dim3 g((a.n + kWavesPB - 1) / kWavesPB), b(64 * kWavesPB);
+  if (cls == 0)
+    hipLaunchKernelGGL((k_lsa_wave<Q, 0>), g, b, 0, s, a);
+  else if (cls == 1)
+    hipLaunchKernelGGL((k_lsa_wave<Q, 1>), g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_lsa_wave<Q, 2>), g, b, 0, s, a);
+}
+
+void launch_lsa_wave(hipStream_t s, Device &, const LsaArgs &a, int sph_model, int cls) {
+  if (a.n == 0) return;
+  if (sph_model == 4)
+    lsa_wave_q<9>(s, a, cls);
+  else if (sph_model == 5)
+    lsa_wave_q<14>(s, a, cls);
+  else
+    lsa_wave_q<20>(s, a, cls);
+}
+
+template <int Q>
+static void pst_wave_q(hipStream_t s, const PstArgs &a, bool tem) {
+  const dim3 g((a.n + kWavesPB - 1) / kWavesPB), b(64 * kWavesPB);
+  if (tem)
+    hipLaunchKernelGGL((k_pst_wave<Q, true>), g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_pst_wave<Q, false>), g, b, 0, s, a);
+}
+
+void launch_pst_wave(hipStream_t s, Device &d, const PstArgs &a, int sph_model) {
+  if (a.n == 0) return;
+  const bool tem = d.opt.has_tem != 0;
+  if (sph_model == 4)
+    pst_wave_q<9>(s, a, tem);
+  else if (sph_model == 5)
+    pst_wave_q<14>(s, a, tem);
+  else
+    pst_wave_q<20>(s, a, tem);
+}
+
+}  // namespace lssph

@@ -17,7 +17,8 @@ enum {
   kFbCount = 3,
   kStrayOverflow = 4,
   kSkinExceeded = 5,
-  kNumCounters = 6
+  kWaveListOverflow = 6,  // a particle of the q = 9..20 kernels with too many neighbors
+  kNumCounters = 7
 };
 
 // Incremental cell rebuild: particles whose new cell is outside the 3x3
@@ -25,6 +26,9 @@ enum {
 // more than kStrayCap of them in one stage is reported as an error.
 constexpr int kStrayCap = 4096;
 constexpr int kArrivals = 8;  // arrivals per new cell merged directly
+// Neighbor-list capacity per particle of the one-wavefront-per-particle
+// kernels (SPH_MODEL 4-6).
+constexpr int kWaveList = 256;
 enum { kFallbacks = 0, kOverflows = 1, kNumStats64 = 2 };
 
 // Raw device pointers and sizes; passed to the launchers.

@@ -1,0 +1,427 @@
+// ls_common.hpp -- pieces shared by the LS-SPH pair operators: field layout,
+// LS sums and bases, the mixed-precision solve, kernel arguments, and the
+// RHS + Runge-Kutta stage epilogue.
+#pragma once
+
+#include <type_traits>
+
+#include "pair_common.hpp"
+
+namespace lssph {
+
+// ---------------------------------------------------------------------------
+// Field layout of the LS type-A right-hand sides.
+//   CLS 0 (diffusion test): u
+//   CLS 1 (flow):           u, v, pre [, rho when delta-SPH is on]
+//   CLS 2 (Boussinesq):     u, v, pre, tem [, rho]
+// rho's column only feeds the delta-SPH term (LSSPH_A_mod.f90:253), so it is
+// carried only when coe_delta_sph != 0.
+// ---------------------------------------------------------------------------
+template <int CLS, bool DELTA>
+struct Layout {
+  static constexpr int U = 0, V = 1, P = 2, T = 3;
+  static constexpr int NF = CLS == 0 ? 1 : (CLS == 1 ? 3 : 4) + (DELTA ? 1 : 0);
+  static constexpr int RHO = NF - 1;  // valid when DELTA
+};
+
+template <int CLS, bool DELTA, typename R>
+__device__ __forceinline__ void field_diffs(const Rec &nb, const Rec &me, R (&df)[Layout<CLS, DELTA>::NF]) {
+  using L = Layout<CLS, DELTA>;
+  df[L::U] = R(nb.u - me.u);
+  if constexpr (CLS >= 1) {
+    df[L::V] = R(nb.v - me.v);
+    df[L::P] = R(nb.pre - me.pre);
+    if constexpr (CLS == 2) df[L::T] = R(nb.tem - me.tem);
+    if constexpr (DELTA) df[L::RHO] = R(nb.rho - me.rho);
+  }
+}
+
+// Moment matrix (upper triangle, index i*(i+1)/2 + r for r <= i) and the
+// right-hand sides of a q-term LS fit: q = 5, or q = 2 for the shifting of
+// the classical SPH models (shift_interpolate_mod.f90:43-44).
+template <typename R, int NF, int Q = 5>
+struct LsSums {
+  static constexpr int NM = Q * (Q + 1) / 2;
+  R M[NM];
+  R b[NF][Q];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int i = 0; i < NM; ++i) M[i] = R(0);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int i = 0; i < Q; ++i) b[f][i] = R(0);
+  }
+  // LSSPH_A_mod.f90:174-190
+  __device__ __forceinline__ void add(R vw, const R (&a)[Q], const R (&df)[NF]) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const R vwa = vw * a[i];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) b[f][i] += vwa * df[f];
+#pragma unroll
+      for (int r = 0; r <= i; ++r) M[i * (i + 1) / 2 + r] += vwa * a[r];
+    }
+  }
+};
+
+// Taylor basis a = [x/h, y/h, x^2/2h^2, y^2/2h^2, xy/h^2] (LSSPH_A_mod.f90:
+// 140-145); ih2 = (1/h)/h as the reference's inv_h(2).
+__device__ __forceinline__ void basis5(double x, double y, double ih, double ih2, double (&a)[5]) {
+  a[0] = x * ih;
+  a[1] = y * ih;
+  a[2] = 0.5 * (x * x) * ih2;
+  a[3] = 0.5 * (y * y) * ih2;
+  a[4] = x * y * ih2;
+}
+
+// The q-term basis: q = 2 keeps the first-order terms (the shifting of the
+// classical models), q = 5 is basis5.
+template <int Q>
+__device__ __forceinline__ void basis_q(double x, double y, double ih, double ih2, double (&a)[Q]) {
+  static_assert(Q == 2 || Q == 5, "q = 2 or 5");
+  if constexpr (Q == 5) {
+    basis5(x, y, ih, ih2, a);
+  } else {
+    a[0] = x * ih;
+    a[1] = y * ih;
+  }
+}
+
+// Same basis from offsets already scaled by 1/h (mixed precision).
+__device__ __forceinline__ void basis5s(float xs, float ys, float (&a)[5]) {
+  a[0] = xs;
+  a[1] = ys;
+  a[2] = 0.5f * (xs * xs);
+  a[3] = 0.5f * (ys * ys);
+  a[4] = xs * ys;
+}
+
+template <typename R, int NF, int Q>
+__device__ __forceinline__ void unpack(const LsSums<R, NF, Q> &S, double (&A)[Q][Q], double (&B)[NF][Q]) {
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+#pragma unroll
+    for (int r = 0; r <= i; ++r) {
+      A[r][i] = double(S.M[i * (i + 1) / 2 + r]);
+      A[i][r] = A[r][i];
+    }
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int i = 0; i < Q; ++i) B[f][i] = double(S.b[f][i]);
+}
+
+// Symmetric 5x5 entry (i, c) of packed upper-triangle sums.
+template <typename R>
+__device__ __forceinline__ double sym5(const R (&M)[15], int i, int c) {
+  return i >= c ? double(M[i * (i + 1) / 2 + c]) : double(M[c * (c + 1) / 2 + i]);
+}
+
+// FP64 LDL^T of the symmetric 5x5 moment matrix held as FP32 sums, with one
+// step of iterative refinement per right-hand side (x += M^-1 (b - M x),
+// the residual formed in FP64 against the FP32 sums).  The right-hand sides
+// are processed one at a time and the sums stay FP32 in registers, which
+// keeps the solve's register footprint near that of the pair loop.  Returns
+// false when a pivot is not positive or the pivot ratio max(d)/min(d) -- a
+// cheap lower bound of cond_2(M) -- exceeds cond_max; the caller then
+// re-solves in full FP64.
+template <int NF>
+__device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, double (&X)[NF][5],
+                                                  double cond_max) {
+  double L[5][5], d[5], dinv[5];
+  double dmax = 0.0, dmin = 1.0e300;
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    double dj = sym5(S.M, j, j);
+#pragma unroll
+    for (int k = 0; k < j; ++k) dj -= L[j][k] * L[j][k] * d[k];
+    ok = ok && (dj > 0.0);
+    d[j] = dj;
+    dinv[j] = 1.0 / dj;
+    dmax = fmax(dmax, dj);
+    dmin = fmin(dmin, dj);
+#pragma unroll
+    for (int i = j + 1; i < 5; ++i) {
+      double t = sym5(S.M, i, j);
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k] * d[k];
+      L[i][j] = t * dinv[j];
+    }
+  }
+  if (!ok || dmax > cond_max * dmin) return false;
+  auto solve = [&](double (&y)[5]) {
+#pragma unroll
+    for (int i = 1; i < 5; ++i)
+#pragma unroll
+      for (int k = 0; k < i; ++k) y[i] -= L[i][k] * y[k];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) y[i] *= dinv[i];
+#pragma unroll
+    for (int i = 3; i >= 0; --i)
+#pragma unroll
+      for (int k = i + 1; k < 5; ++k) y[i] -= L[k][i] * y[k];
+  };
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    double x[5], r[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) x[i] = double(S.b[f][i]);
+    solve(x);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      double t = double(S.b[f][i]);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) t -= sym5(S.M, i, c) * x[c];
+      r[i] = t;
+    }
+    solve(r);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) X[f][i] = x[i] + r[i];
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// LS type A + stage update
+// ---------------------------------------------------------------------------
+struct LsaArgs {
+  const Rec *__restrict__ mir;        // records in slot order (read)
+  Rec *__restrict__ mout;             // records in slot order (write)
+  const int32_t *__restrict__ sinfo;  // info in slot order
+  const int32_t *__restrict__ items;  // slot -> physical id
+  const int32_t *__restrict__ orig_of;  // physical id -> upload index
+  const int32_t *__restrict__ cstart;
+  const InstDev *__restrict__ inst;
+  double *old_[6];
+  double *new_[6];
+  double *rhs[4];
+  unsigned long long *stats64;
+  int32_t *counters;
+  int32_t *fb_list;
+  int32_t *fb_count;
+  MoveOut mv;
+  int n, kt, list_len, mode;
+};
+
+template <int NF>
+__device__ __forceinline__ void scale_derivs(const double (&X)[NF][5], const InstDev &I, double (&d)[NF][5]) {
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    d[f][0] = X[f][0] * I.inv_h;
+    d[f][1] = X[f][1] * I.inv_h;
+    d[f][2] = X[f][2] * I.inv_h2;
+    d[f][3] = X[f][3] * I.inv_h2;
+    d[f][4] = X[f][4] * I.inv_h2;
+  }
+}
+
+// FP64 assembly + DGESV-semantics solve; d[f] = (d_x, d_y, d_xx, d_yy, d_xy)
+// (LSSPH_A_mod.f90:199-240).
+template <int CLS, bool DELTA, int KT>
+__device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
+                                         const int32_t *lds, int cnt, int list_len, int B, int tid,
+                                         double (&d)[Layout<CLS, DELTA>::NF][5]) {
+  constexpr int NF = Layout<CLS, DELTA>::NF;
+  const int kt = A.kt;
+  LsSums<double, NF> S;
+  S.zero();
+  walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, list_len, B, tid,
+                 [&](const Rec &nb, double dx, double dy) {
+                   const double r = sqrt(sq_dist(dx, dy));
+                   const double w = I.w_norm * kshape_w<double, KT>(kt, r * I.inv_h);
+                   double a[5];
+                   basis5(dx, dy, I.inv_h, I.inv_h2, a);
+                   double df[NF];
+                   field_diffs<CLS, DELTA>(nb, me, df);
+                   S.add(nb.vol * w, a, df);
+                 });
+  double M[5][5], X[NF][5];
+  unpack(S, M, X);
+  lu_solve<5, NF>(M, X);
+  scale_derivs<NF>(X, I, d);
+}
+
+// Mixed-precision assembly + FP64 LDL^T with refinement; false -> fallback.
+template <int CLS, bool DELTA, int KT>
+__device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
+                                         const int32_t *lds, int cnt, int list_len, int B, int tid,
+                                         double (&d)[Layout<CLS, DELTA>::NF][5]) {
+  constexpr int NF = Layout<CLS, DELTA>::NF;
+  const int kt = A.kt;
+  LsSums<float, NF> S;
+  S.zero();
+  walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, list_len, B, tid,
+                 [&](const Rec &nb, double dx, double dy) {
+                   const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
+                   const float q = fast_sqrtf(xs * xs + ys * ys);
+                   const float vw = float(nb.vol * I.w_norm) * kshape_w<float, KT>(kt, q);
+                   float a[5];
+                   basis5s(xs, ys, a);
+                   float df[NF];
+                   field_diffs<CLS, DELTA>(nb, me, df);
+                   S.add(vw, a, df);
+                 });
+  double X[NF][5];
+  if (!ldlt_solve_refine<NF>(S, X, I.cond_max)) return false;
+  scale_derivs<NF>(X, I, d);
+  return true;
+}
+
+// RHS of the governing equations (LSSPH_A_mod.f90:246-268) and the stage
+// update (RK2_mod.f90:77-107, RK4_mod.f90:216-286) of physical particle p.
+template <int CLS, bool DELTA>
+__device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p, int s, const Rec &me,
+                                       const double (&d)[Layout<CLS, DELTA>::NF][5]) {
+  using L = Layout<CLS, DELTA>;
+  double r_u, r_v = 0.0, r_rho = 0.0, r_tem = 0.0;
+  if constexpr (CLS == 0) {
+    r_u = I.diff_coef * (d[L::U][2] + d[L::U][3]) +
+          I.diff_source * (2.0 * kPi * kPi * sin(kPi * (me.x - I.wl_thick)) * cos(kPi * (me.y - I.wl_thick)));
+  } else {
+    double drho_lap = 0.0;
+    if constexpr (DELTA) drho_lap = d[L::RHO][2] + d[L::RHO][3];
+    r_rho = -me.rho * (d[L::U][0] + d[L::V][1]) + I.delta * drho_lap;
+    r_u = -d[L::P][0] / me.rho + (I.vis_ref / me.rho) * (d[L::U][2] + d[L::U][3]);
+    r_v = -d[L::P][1] / me.rho + (I.vis_ref / me.rho) * (d[L::V][2] + d[L::V][3]);
+    r_rho = r_rho / I.zeta_sq;
+    if constexpr (CLS == 2) {
+      r_v = r_v + I.alpha_ref * (me.tem - I.tem_ave) * I.gravity;
+      r_u = r_u / I.xi_sq;
+      r_v = r_v / I.xi_sq;
+      r_tem = I.thermal_dif * (d[L::T][2] + d[L::T][3]);
+    }
+  }
+  const int mode = A.mode;
+  if (mode == kRhsOnly) {
+    A.rhs[0][p] = r_u;
+    A.rhs[1][p] = r_v;
+    A.rhs[2][p] = r_rho;
+    A.rhs[3][p] = r_tem;
+    return;
+  }
+  constexpr bool DIFF = (CLS == 0);
+  constexpr bool T = (CLS == 2);
+  const bool per = I.periodic;
+  const double dt = I.dt;
+  Rec out = me;
+  if (mode == kRk2S1 || mode == kRk2S2) {
+    const double dte = (mode == kRk2S1) ? dt / 2.0 : dt;
+    double ox, oy, ou, ov, orho, otem;
+    if (mode == kRk2S1) {
+      ox = me.x; oy = me.y; ou = me.u; ov = me.v; orho = me.rho; otem = me.tem;
+      A.old_[2][p] = ou;
+      if (!DIFF) {
+        A.old_[0][p] = ox; A.old_[1][p] = oy; A.old_[3][p] = ov; A.old_[4][p] = orho;
+      }
+      if (T) A.old_[5][p] = otem;
+    } else {
+      ou = A.old_[2][p];
+      ox = oy = ov = orho = otem = 0.0;
+      if (!DIFF) {
+        ox = A.old_[0][p]; oy = A.old_[1][p]; ov = A.old_[3][p]; orho = A.old_[4][p];
+      }
+      if (T) otem = A.old_[5][p];
+    }
+    if (!DIFF) {
+      out.x = wrap_pos(ox + me.u * dte, I.len_x, per);
+      out.y = wrap_pos(oy + me.v * dte, I.len_y, per);
+      out.v = ov + r_v * dte;
+      out.rho = orho + r_rho * dte;
+    }
+    out.u = ou + r_u * dte;
+    if (T) out.tem = otem + r_tem * dte;
+  } else {
+    const int st = mode - kRk4S1 + 1;
+    const double dt1 = (st == 1 || st == 4) ? dt / 6.0 : dt / 3.0;
+    const double dt2 = (st <= 2) ? dt / 2.0 : (st == 3 ? dt : 0.0);
+    double nx_, ny_, nu_, nv_, nrho_, ntem_;
+    if (st == 1) {
+      A.old_[2][p] = me.u;
+      if (!DIFF) {
+        A.old_[0][p] = me.x; A.old_[1][p] = me.y; A.old_[3][p] = me.v; A.old_[4][p] = me.rho;
+      }
+      if (T) A.old_[5][p] = me.tem;
+      nx_ = me.x; ny_ = me.y; nu_ = me.u; nv_ = me.v; nrho_ = me.rho; ntem_ = me.tem;
+    } else {
+      nu_ = A.new_[2][p];
+      nx_ = ny_ = nv_ = nrho_ = ntem_ = 0.0;
+      if (!DIFF) {
+        nx_ = A.new_[0][p]; ny_ = A.new_[1][p]; nv_ = A.new_[3][p]; nrho_ = A.new_[4][p];
+      }
+      if (T) ntem_ = A.new_[5][p];
+    }
+    nu_ = nu_ + r_u * dt1;
+    if (!DIFF) {
+      nv_ = nv_ + r_v * dt1;
+      nx_ = nx_ + me.u * dt1;
+      ny_ = ny_ + me.v * dt1;
+      nrho_ = nrho_ + r_rho * dt1;
+    }
+    if (T) ntem_ = ntem_ + r_tem * dt1;
+    if (st < 4) {
+      A.new_[2][p] = nu_;
+      if (!DIFF) {
+        A.new_[0][p] = nx_; A.new_[1][p] = ny_; A.new_[3][p] = nv_; A.new_[4][p] = nrho_;
+      }
+      if (T) A.new_[5][p] = ntem_;
+      double ox, oy, ou, ov, orho, otem;
+      if (st == 1) {
+        ox = me.x; oy = me.y; ou = me.u; ov = me.v; orho = me.rho; otem = me.tem;
+      } else {
+        ou = A.old_[2][p];
+        ox = oy = ov = orho = otem = 0.0;
+        if (!DIFF) {
+          ox = A.old_[0][p]; oy = A.old_[1][p]; ov = A.old_[3][p]; orho = A.old_[4][p];
+        }
+        if (T) otem = A.old_[5][p];
+      }
+      if (!DIFF) {
+        out.x = wrap_pos(ox + me.u * dt2, I.len_x, per);
+        out.y = wrap_pos(oy + me.v * dt2, I.len_y, per);
+        out.v = ov + r_v * dt2;
+        out.rho = orho + r_rho * dt2;
+      }
+      out.u = ou + r_u * dt2;
+      if (T) out.tem = otem + r_tem * dt2;
+    } else {
+      out.u = nu_;
+      if (!DIFF) {
+        out.v = nv_;
+        out.x = wrap_pos(nx_, I.len_x, per);
+        out.y = wrap_pos(ny_, I.len_y, per);
+        out.rho = nrho_;
+      }
+      if (T) out.tem = ntem_;
+    }
+  }
+  if (!DIFF) {
+    out.pre = I.c_sq * (out.rho - I.rho_ref);  // equation of state
+    out.vol = I.sp_mass / out.rho;
+  }
+  A.mout[s] = out;
+  if (!DIFF) record_move(A.mv, I, s, out.x, out.y);
+}
+
+// Arguments of the particle-shifting kernels.
+struct PstArgs {
+  const Rec *__restrict__ mir;
+  Rec *__restrict__ mout;
+  const int32_t *__restrict__ sinfo;
+  const int32_t *__restrict__ items;
+  const int32_t *__restrict__ orig_of;
+  const int32_t *__restrict__ cstart;
+  const InstDev *__restrict__ inst;
+  const unsigned long long *__restrict__ umax;
+  unsigned long long *stats64;
+  MoveOut mv;
+  int n, kt, list_len;
+};
+
+// The LS operators of SPH_MODEL 4-6 (q = 9, 14, 20), one wavefront per
+// particle (highorder.hip).  cls: 0 diffusion, 1 flow, 2 Boussinesq.
+void launch_lsa_wave(hipStream_t s, Device &d, const LsaArgs &a, int sph_model, int cls);
+void launch_pst_wave(hipStream_t s, Device &d, const PstArgs &a, int sph_model);
+
+}  // namespace lssph

@@ -17,390 +17,9 @@
 //   LDL^T + one refinement step; particles whose LDL^T pivots are not
 //   positive or whose pivot ratio exceeds cond_max are queued and redone by
 //   the FP64 path in a second, list-driven launch.
-#include <type_traits>
-
-#include "pair_common.hpp"
+#include "ls_common.hpp"
 
 namespace lssph {
-
-// ---------------------------------------------------------------------------
-// Field layout of the LS type-A right-hand sides.
-//   CLS 0 (diffusion test): u
-//   CLS 1 (flow):           u, v, pre [, rho when delta-SPH is on]
-//   CLS 2 (Boussinesq):     u, v, pre, tem [, rho]
-// rho's column only feeds the delta-SPH term (LSSPH_A_mod.f90:253), so it is
-// carried only when coe_delta_sph != 0.
-// ---------------------------------------------------------------------------
-template <int CLS, bool DELTA>
-struct Layout {
-  static constexpr int U = 0, V = 1, P = 2, T = 3;
-  static constexpr int NF = CLS == 0 ? 1 : (CLS == 1 ? 3 : 4) + (DELTA ? 1 : 0);
-  static constexpr int RHO = NF - 1;  // valid when DELTA
-};
-
-template <int CLS, bool DELTA, typename R>
-__device__ __forceinline__ void field_diffs(const Rec &nb, const Rec &me, R (&df)[Layout<CLS, DELTA>::NF]) {
-  using L = Layout<CLS, DELTA>;
-  df[L::U] = R(nb.u - me.u);
-  if constexpr (CLS >= 1) {
-    df[L::V] = R(nb.v - me.v);
-    df[L::P] = R(nb.pre - me.pre);
-    if constexpr (CLS == 2) df[L::T] = R(nb.tem - me.tem);
-    if constexpr (DELTA) df[L::RHO] = R(nb.rho - me.rho);
-  }
-}
-
-// Moment matrix (upper triangle, index i*(i+1)/2 + r for r <= i) and the
-// right-hand sides of the q = 5 LS fit.
-template <typename R, int NF>
-struct LsSums {
-  R M[15];
-  R b[NF][5];
-  __device__ __forceinline__ void zero() {
-#pragma unroll
-    for (int i = 0; i < 15; ++i) M[i] = R(0);
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int i = 0; i < 5; ++i) b[f][i] = R(0);
-  }
-  // LSSPH_A_mod.f90:174-190
-  __device__ __forceinline__ void add(R vw, const R (&a)[5], const R (&df)[NF]) {
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-      const R vwa = vw * a[i];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) b[f][i] += vwa * df[f];
-#pragma unroll
-      for (int r = 0; r <= i; ++r) M[i * (i + 1) / 2 + r] += vwa * a[r];
-    }
-  }
-};
-
-// Taylor basis a = [x/h, y/h, x^2/2h^2, y^2/2h^2, xy/h^2] (LSSPH_A_mod.f90:
-// 140-145); ih2 = (1/h)/h as the reference's inv_h(2).
-__device__ __forceinline__ void basis5(double x, double y, double ih, double ih2, double (&a)[5]) {
-  a[0] = x * ih;
-  a[1] = y * ih;
-  a[2] = 0.5 * (x * x) * ih2;
-  a[3] = 0.5 * (y * y) * ih2;
-  a[4] = x * y * ih2;
-}
-
-// Same basis from offsets already scaled by 1/h (mixed precision).
-__device__ __forceinline__ void basis5s(float xs, float ys, float (&a)[5]) {
-  a[0] = xs;
-  a[1] = ys;
-  a[2] = 0.5f * (xs * xs);
-  a[3] = 0.5f * (ys * ys);
-  a[4] = xs * ys;
-}
-
-template <typename R, int NF>
-__device__ __forceinline__ void unpack(const LsSums<R, NF> &S, double (&A)[5][5], double (&B)[NF][5]) {
-#pragma unroll
-  for (int i = 0; i < 5; ++i)
-#pragma unroll
-    for (int r = 0; r <= i; ++r) {
-      A[r][i] = double(S.M[i * (i + 1) / 2 + r]);
-      A[i][r] = A[r][i];
-    }
-#pragma unroll
-  for (int f = 0; f < NF; ++f)
-#pragma unroll
-    for (int i = 0; i < 5; ++i) B[f][i] = double(S.b[f][i]);
-}
-
-// Symmetric 5x5 entry (i, c) of packed upper-triangle sums.
-template <typename R>
-__device__ __forceinline__ double sym5(const R (&M)[15], int i, int c) {
-  return i >= c ? double(M[i * (i + 1) / 2 + c]) : double(M[c * (c + 1) / 2 + i]);
-}
-
-// FP64 LDL^T of the symmetric 5x5 moment matrix held as FP32 sums, with one
-// step of iterative refinement per right-hand side (x += M^-1 (b - M x),
-// the residual formed in FP64 against the FP32 sums).  The right-hand sides
-// are processed one at a time and the sums stay FP32 in registers, which
-// keeps the solve's register footprint near that of the pair loop.  Returns
-// false when a pivot is not positive or the pivot ratio max(d)/min(d) -- a
-// cheap lower bound of cond_2(M) -- exceeds cond_max; the caller then
-// re-solves in full FP64.
-template <int NF>
-__device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, double (&X)[NF][5],
-                                                  double cond_max) {
-  double L[5][5], d[5], dinv[5];
-  double dmax = 0.0, dmin = 1.0e300;
-  bool ok = true;
-#pragma unroll
-  for (int j = 0; j < 5; ++j) {
-    double dj = sym5(S.M, j, j);
-#pragma unroll
-    for (int k = 0; k < j; ++k) dj -= L[j][k] * L[j][k] * d[k];
-    ok = ok && (dj > 0.0);
-    d[j] = dj;
-    dinv[j] = 1.0 / dj;
-    dmax = fmax(dmax, dj);
-    dmin = fmin(dmin, dj);
-#pragma unroll
-    for (int i = j + 1; i < 5; ++i) {
-      double t = sym5(S.M, i, j);
-#pragma unroll
-      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k] * d[k];
-      L[i][j] = t * dinv[j];
-    }
-  }
-  if (!ok || dmax > cond_max * dmin) return false;
-  auto solve = [&](double (&y)[5]) {
-#pragma unroll
-    for (int i = 1; i < 5; ++i)
-#pragma unroll
-      for (int k = 0; k < i; ++k) y[i] -= L[i][k] * y[k];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) y[i] *= dinv[i];
-#pragma unroll
-    for (int i = 3; i >= 0; --i)
-#pragma unroll
-      for (int k = i + 1; k < 5; ++k) y[i] -= L[k][i] * y[k];
-  };
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    double x[5], r[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) x[i] = double(S.b[f][i]);
-    solve(x);
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-      double t = double(S.b[f][i]);
-#pragma unroll
-      for (int c = 0; c < 5; ++c) t -= sym5(S.M, i, c) * x[c];
-      r[i] = t;
-    }
-    solve(r);
-#pragma unroll
-    for (int i = 0; i < 5; ++i) X[f][i] = x[i] + r[i];
-  }
-  return true;
-}
-
-// ---------------------------------------------------------------------------
-// LS type A + stage update
-// ---------------------------------------------------------------------------
-struct LsaArgs {
-  const Rec *__restrict__ mir;        // records in slot order (read)
-  Rec *__restrict__ mout;             // records in slot order (write)
-  const int32_t *__restrict__ sinfo;  // info in slot order
-  const int32_t *__restrict__ items;  // slot -> physical id
-  const int32_t *__restrict__ cstart;
-  const InstDev *__restrict__ inst;
-  double *old_[6];
-  double *new_[6];
-  double *rhs[4];
-  unsigned long long *stats64;
-  int32_t *counters;
-  int32_t *fb_list;
-  int32_t *fb_count;
-  MoveOut mv;
-  int n, kt, list_len, mode;
-};
-
-template <int NF>
-__device__ __forceinline__ void scale_derivs(const double (&X)[NF][5], const InstDev &I, double (&d)[NF][5]) {
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    d[f][0] = X[f][0] * I.inv_h;
-    d[f][1] = X[f][1] * I.inv_h;
-    d[f][2] = X[f][2] * I.inv_h2;
-    d[f][3] = X[f][3] * I.inv_h2;
-    d[f][4] = X[f][4] * I.inv_h2;
-  }
-}
-
-// FP64 assembly + DGESV-semantics solve; d[f] = (d_x, d_y, d_xx, d_yy, d_xy)
-// (LSSPH_A_mod.f90:199-240).
-template <int CLS, bool DELTA, int KT>
-__device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
-                                         const int32_t *lds, int cnt, int list_len, int B, int tid,
-                                         double (&d)[Layout<CLS, DELTA>::NF][5]) {
-  constexpr int NF = Layout<CLS, DELTA>::NF;
-  const int kt = A.kt;
-  LsSums<double, NF> S;
-  S.zero();
-  walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, list_len, B, tid,
-                 [&](const Rec &nb, double dx, double dy) {
-                   const double r = sqrt(sq_dist(dx, dy));
-                   const double w = I.w_norm * kshape_w<double, KT>(kt, r * I.inv_h);
-                   double a[5];
-                   basis5(dx, dy, I.inv_h, I.inv_h2, a);
-                   double df[NF];
-                   field_diffs<CLS, DELTA>(nb, me, df);
-                   S.add(nb.vol * w, a, df);
-                 });
-  double M[5][5], X[NF][5];
-  unpack(S, M, X);
-  lu_solve<5, NF>(M, X);
-  scale_derivs<NF>(X, I, d);
-}
-
-// Mixed-precision assembly + FP64 LDL^T with refinement; false -> fallback.
-template <int CLS, bool DELTA, int KT>
-__device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
-                                         const int32_t *lds, int cnt, int list_len, int B, int tid,
-                                         double (&d)[Layout<CLS, DELTA>::NF][5]) {
-  constexpr int NF = Layout<CLS, DELTA>::NF;
-  const int kt = A.kt;
-  LsSums<float, NF> S;
-  S.zero();
-  walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, list_len, B, tid,
-                 [&](const Rec &nb, double dx, double dy) {
-                   const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
-                   const float q = fast_sqrtf(xs * xs + ys * ys);
-                   const float vw = float(nb.vol * I.w_norm) * kshape_w<float, KT>(kt, q);
-                   float a[5];
-                   basis5s(xs, ys, a);
-                   float df[NF];
-                   field_diffs<CLS, DELTA>(nb, me, df);
-                   S.add(vw, a, df);
-                 });
-  double X[NF][5];
-  if (!ldlt_solve_refine<NF>(S, X, I.cond_max)) return false;
-  scale_derivs<NF>(X, I, d);
-  return true;
-}
-
-// RHS of the governing equations (LSSPH_A_mod.f90:246-268) and the stage
-// update (RK2_mod.f90:77-107, RK4_mod.f90:216-286) of physical particle p.
-template <int CLS, bool DELTA>
-__device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p, int s, const Rec &me,
-                                       const double (&d)[Layout<CLS, DELTA>::NF][5]) {
-  using L = Layout<CLS, DELTA>;
-  double r_u, r_v = 0.0, r_rho = 0.0, r_tem = 0.0;
-  if constexpr (CLS == 0) {
-    r_u = I.diff_coef * (d[L::U][2] + d[L::U][3]) +
-          I.diff_source * (2.0 * kPi * kPi * sin(kPi * (me.x - I.wl_thick)) * cos(kPi * (me.y - I.wl_thick)));
-  } else {
-    double drho_lap = 0.0;
-    if constexpr (DELTA) drho_lap = d[L::RHO][2] + d[L::RHO][3];
-    r_rho = -me.rho * (d[L::U][0] + d[L::V][1]) + I.delta * drho_lap;
-    r_u = -d[L::P][0] / me.rho + (I.vis_ref / me.rho) * (d[L::U][2] + d[L::U][3]);
-    r_v = -d[L::P][1] / me.rho + (I.vis_ref / me.rho) * (d[L::V][2] + d[L::V][3]);
-    r_rho = r_rho / I.zeta_sq;
-    if constexpr (CLS == 2) {
-      r_v = r_v + I.alpha_ref * (me.tem - I.tem_ave) * I.gravity;
-      r_u = r_u / I.xi_sq;
-      r_v = r_v / I.xi_sq;
-      r_tem = I.thermal_dif * (d[L::T][2] + d[L::T][3]);
-    }
-  }
-  const int mode = A.mode;
-  if (mode == kRhsOnly) {
-    A.rhs[0][p] = r_u;
-    A.rhs[1][p] = r_v;
-    A.rhs[2][p] = r_rho;
-    A.rhs[3][p] = r_tem;
-    return;
-  }
-  constexpr bool DIFF = (CLS == 0);
-  constexpr bool T = (CLS == 2);
-  const bool per = I.periodic;
-  const double dt = I.dt;
-  Rec out = me;
-  if (mode == kRk2S1 || mode == kRk2S2) {
-    const double dte = (mode == kRk2S1) ? dt / 2.0 : dt;
-    double ox, oy, ou, ov, orho, otem;
-    if (mode == kRk2S1) {
-      ox = me.x; oy = me.y; ou = me.u; ov = me.v; orho = me.rho; otem = me.tem;
-      A.old_[2][p] = ou;
-      if (!DIFF) {
-        A.old_[0][p] = ox; A.old_[1][p] = oy; A.old_[3][p] = ov; A.old_[4][p] = orho;
-      }
-      if (T) A.old_[5][p] = otem;
-    } else {
-      ou = A.old_[2][p];
-      ox = oy = ov = orho = otem = 0.0;
-      if (!DIFF) {
-        ox = A.old_[0][p]; oy = A.old_[1][p]; ov = A.old_[3][p]; orho = A.old_[4][p];
-      }
-      if (T) otem = A.old_[5][p];
-    }
-    if (!DIFF) {
-      out.x = wrap_pos(ox + me.u * dte, I.len_x, per);
-      out.y = wrap_pos(oy + me.v * dte, I.len_y, per);
-      out.v = ov + r_v * dte;
-      out.rho = orho + r_rho * dte;
-    }
-    out.u = ou + r_u * dte;
-    if (T) out.tem = otem + r_tem * dte;
-  } else {
-    const int st = mode - kRk4S1 + 1;
-    const double dt1 = (st == 1 || st == 4) ? dt / 6.0 : dt / 3.0;
-    const double dt2 = (st <= 2) ? dt / 2.0 : (st == 3 ? dt : 0.0);
-    double nx_, ny_, nu_, nv_, nrho_, ntem_;
-    if (st == 1) {
-      A.old_[2][p] = me.u;
-      if (!DIFF) {
-        A.old_[0][p] = me.x; A.old_[1][p] = me.y; A.old_[3][p] = me.v; A.old_[4][p] = me.rho;
-      }
-      if (T) A.old_[5][p] = me.tem;
-      nx_ = me.x; ny_ = me.y; nu_ = me.u; nv_ = me.v; nrho_ = me.rho; ntem_ = me.tem;
-    } else {
-      nu_ = A.new_[2][p];
-      nx_ = ny_ = nv_ = nrho_ = ntem_ = 0.0;
-      if (!DIFF) {
-        nx_ = A.new_[0][p]; ny_ = A.new_[1][p]; nv_ = A.new_[3][p]; nrho_ = A.new_[4][p];
-      }
-      if (T) ntem_ = A.new_[5][p];
-    }
-    nu_ = nu_ + r_u * dt1;
-    if (!DIFF) {
-      nv_ = nv_ + r_v * dt1;
-      nx_ = nx_ + me.u * dt1;
-      ny_ = ny_ + me.v * dt1;
-      nrho_ = nrho_ + r_rho * dt1;
-    }
-    if (T) ntem_ = ntem_ + r_tem * dt1;
-    if (st < 4) {
-      A.new_[2][p] = nu_;
-      if (!DIFF) {
-        A.new_[0][p] = nx_; A.new_[1][p] = ny_; A.new_[3][p] = nv_; A.new_[4][p] = nrho_;
-      }
-      if (T) A.new_[5][p] = ntem_;
-      double ox, oy, ou, ov, orho, otem;
-      if (st == 1) {
-        ox = me.x; oy = me.y; ou = me.u; ov = me.v; orho = me.rho; otem = me.tem;
-      } else {
-        ou = A.old_[2][p];
-        ox = oy = ov = orho = otem = 0.0;
-        if (!DIFF) {
-          ox = A.old_[0][p]; oy = A.old_[1][p]; ov = A.old_[3][p]; orho = A.old_[4][p];
-        }
-        if (T) otem = A.old_[5][p];
-      }
-      if (!DIFF) {
-        out.x = wrap_pos(ox + me.u * dt2, I.len_x, per);
-        out.y = wrap_pos(oy + me.v * dt2, I.len_y, per);
-        out.v = ov + r_v * dt2;
-        out.rho = orho + r_rho * dt2;
-      }
-      out.u = ou + r_u * dt2;
-      if (T) out.tem = otem + r_tem * dt2;
-    } else {
-      out.u = nu_;
-      if (!DIFF) {
-        out.v = nv_;
-        out.x = wrap_pos(nx_, I.len_x, per);
-        out.y = wrap_pos(ny_, I.len_y, per);
-        out.rho = nrho_;
-      }
-      if (T) out.tem = ntem_;
-    }
-  }
-  if (!DIFF) {
-    out.pre = I.c_sq * (out.rho - I.rho_ref);  // equation of state
-    out.vol = I.sp_mass / out.rho;
-  }
-  A.mout[s] = out;
-  if (!DIFF) record_move(A.mv, I, s, out.x, out.y);
-}
 
 template <int CLS, bool MIXED, bool DELTA, int KT>
 __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
@@ -465,23 +84,87 @@ __global__ void __launch_bounds__(256) k_lsa_fallback(LsaArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// Classical SPH (classical_SPH_mod.f90:26-170), SPH_MODEL 1 (SUM: pressure
+// gradient over p_j + p_i) and 2 (p_j - p_i), FP64, with the stage update of
+// k_lsa.  The reference's x_ij = x_i - x_j is the exact negation of the
+// offsets the walk passes; the kernel-gradient sums are kept as the
+// reference forms them and handed to the epilogue as derivatives:
+// d[U][0] = du/dx, d[V][1] = dv/dy, d[P][0..1] = grad p, d[*][2] = Laplacian.
+// ---------------------------------------------------------------------------
+template <int CLS, bool DELTA, bool SUM>
+__global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
+  using L = Layout<CLS, DELTA>;
+  constexpr int NF = L::NF;
+  extern __shared__ int32_t lds[];
+  const int tid = threadIdx.x, B = blockDim.x;
+  const int s = xcd_block() * B + tid;
+  if (s >= A.n) return;
+  const int32_t inf = A.sinfo[s];
+  if (info_ptype(inf) != 0) {
+    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];
+    return;
+  }
+  const InstDev &I = A.inst[info_inst(inf)];
+  const Rec me = A.mir[s];
+  const int lc = local_cell(I, me.x, me.y);
+  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
+  if (cnt > A.list_len) {
+    atomicAdd(&A.stats64[kOverflows], 1ull);
+    atomicMax(&A.counters[kMaxNbr], cnt - 1);
+  }
+  const int kt = A.kt;
+  double du_dx = 0.0, dv_dy = 0.0, dp_dx = 0.0, dp_dy = 0.0;
+  double lap_u = 0.0, lap_v = 0.0, lap_t = 0.0, lap_rho = 0.0;
+  walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, A.list_len, B, tid,
+                 [&](const Rec &nb, double dx, double dy) {
+                   const double r = sqrt(sq_dist(dx, dy));
+                   if (!(r > 0.0)) return;  // the particle itself
+                   const double x_ij = -dx, y_ij = -dy;
+                   const double dwdr = I.w_norm * kshape_dw<double, 0>(kt, r * I.inv_h) * I.dw_fac;
+                   const double vj = nb.vol;
+                   const double dwx = dwdr * x_ij / r * vj;  // :103-104
+                   const double dwy = dwdr * y_ij / r * vj;
+                   const double g = 2.0 * vj * dwdr / r;     // :124-134
+                   du_dx = du_dx + dwx * (nb.u - me.u);
+                   lap_u = lap_u + g * (me.u - nb.u);
+                   if constexpr (CLS >= 1) {
+                     dv_dy = dv_dy + dwy * (nb.v - me.v);
+                     const double pp = SUM ? nb.pre + me.pre : nb.pre - me.pre;
+                     dp_dx = dp_dx + dwx * pp;
+                     dp_dy = dp_dy + dwy * pp;
+                     lap_v = lap_v + g * (me.v - nb.v);
+                   }
+                   if constexpr (CLS == 2) lap_t = lap_t + g * (me.tem - nb.tem);
+                   if constexpr (DELTA) lap_rho = lap_rho + g * (me.rho - nb.rho);
+                 });
+  double d[NF][5];
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int i = 0; i < 5; ++i) d[f][i] = 0.0;
+  d[L::U][0] = du_dx;
+  d[L::U][2] = lap_u;
+  if constexpr (CLS >= 1) {
+    d[L::V][1] = dv_dy;
+    d[L::V][2] = lap_v;
+    d[L::P][0] = dp_dx;
+    d[L::P][1] = dp_dy;
+  }
+  if constexpr (CLS == 2) d[L::T][2] = lap_t;
+  if constexpr (DELTA) d[L::RHO][2] = lap_rho;
+  __asm__ volatile("" ::: "memory");
+  const Rec me2 = A.mir[s];
+  finish<CLS, DELTA>(A, I, A.items[s], s, me2, d);
+}
+
+// ---------------------------------------------------------------------------
 // Particle shifting (PST_mod.f90:37-114, shift_interpolate_mod.f90:23-300)
 // ---------------------------------------------------------------------------
-struct PstArgs {
-  const Rec *__restrict__ mir;
-  Rec *__restrict__ mout;
-  const int32_t *__restrict__ sinfo;
-  const int32_t *__restrict__ items;
-  const int32_t *__restrict__ cstart;
-  const InstDev *__restrict__ inst;
-  const unsigned long long *__restrict__ umax;
-  unsigned long long *stats64;
-  MoveOut mv;
-  int n, kt, list_len;
-};
 
-template <bool T, bool MIXED, int KT>
+// Q = 5 for LS-SPH (SPH_MODEL 3), Q = 2 for the classical models.
+template <bool T, bool MIXED, int KT, int Q>
 __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
+  static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
   constexpr int NF = T ? 4 : 3;  // u, v, rho (, tem)
   using R = typename std::conditional<MIXED, float, double>::type;
   extern __shared__ int32_t lds[];
@@ -502,7 +185,7 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   const int cnt = scan_candidates(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
   if (cnt > A.list_len) atomicAdd(&A.stats64[kOverflows], 1ull);
 
-  LsSums<R, NF> S;
+  LsSums<R, NF, Q> S;
   S.zero();
   R sx = R(0), sy = R(0);
   const double inv_w_ave = 1.0 / I.w_ave;
@@ -527,8 +210,8 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
                      const double f = 1.0 + 0.2 * ((wr * wr) * (wr * wr));
                      sx = sx + f * dwx;
                      sy = sy + f * dwy;
-                     double a[5];
-                     basis5(dx, dy, I.inv_h, I.inv_h2, a);
+                     double a[Q];
+                     basis_q<Q>(dx, dy, I.inv_h, I.inv_h2, a);
                      S.add(vj * w, a, df);
                    } else {
                      const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
@@ -557,32 +240,32 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
     dys = (0.2 * I.h) * dys / rs;
   }
   // LS correction of u, v, rho (, tem) to the shifted position (:352-415)
-  double X[NF][5];
-  if (cnt - 1 >= 5) {  // shift_interpolate_mod.f90:240 (num_neighbors >= q)
+  double X[NF][Q];
+  if (cnt - 1 >= Q) {  // shift_interpolate_mod.f90:240 (num_neighbors >= q)
     bool solved = false;
     if constexpr (MIXED) {
       solved = ldlt_solve_refine<NF>(S, X, I.cond_max);
       if (!solved) atomicAdd(&A.stats64[kFallbacks], 1ull);
     }
     if (!solved) {
-      double M[5][5];
+      double M[Q][Q];
       unpack(S, M, X);
-      lu_solve<5, NF>(M, X);
+      lu_solve<Q, NF>(M, X);
     }
   } else {
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
-      for (int i = 0; i < 5; ++i) X[f][i] = 0.0;
+      for (int i = 0; i < Q; ++i) X[f][i] = 0.0;
   }
-  double a[5];
-  basis5(dxs, dys, I.inv_h, I.inv_h2, a);
+  double a[Q];
+  basis_q<Q>(dxs, dys, I.inv_h, I.inv_h2, a);
   double acc[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     double t = 0.0;
 #pragma unroll
-    for (int i = 0; i < 5; ++i) t += a[i] * X[f][i];
+    for (int i = 0; i < Q; ++i) t += a[i] * X[f][i];
     acc[f] = t;
   }
   Rec out = me;
@@ -656,6 +339,15 @@ __global__ void __launch_bounds__(256) k_dfmax(const Rec *__restrict__ rec, cons
 // trash entry of the scan.
 static size_t list_lds_bytes(int list_len, int block) { return (size_t)4 * (list_len + 1) * block; }
 
+template <int CLS, bool DELTA>
+static void csph_launch(hipStream_t s, const LsaArgs &a, int block, size_t lds, bool sum) {
+  const dim3 g(xcd_grid(a.n, block)), b(block);
+  if (sum)
+    hipLaunchKernelGGL((k_csph<CLS, DELTA, true>), g, b, lds, s, a);
+  else
+    hipLaunchKernelGGL((k_csph<CLS, DELTA, false>), g, b, lds, s, a);
+}
+
 template <int CLS, bool MIXED, bool DELTA, int KT>
 static void lsa_launch(hipStream_t s, const LsaArgs &a, int block, size_t lds) {
   hipLaunchKernelGGL((k_lsa<CLS, MIXED, DELTA, KT>), dim3(xcd_grid(a.n, block)), dim3(block), lds, s, a);
@@ -685,6 +377,7 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
   a.mout = d.mir[cur ^ 1];
   a.sinfo = d.sinfo;
   a.items = d.items;
+  a.orig_of = d.orig_of;
   a.cstart = d.cell_start;
   a.inst = d.inst;
   for (int f = 0; f < 6; ++f) {
@@ -702,11 +395,26 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
   a.list_len = d.list_len;
   a.mode = mode;
   const size_t lds = list_lds_bytes(d.list_len, d.block);
-  const bool mixed = d.opt.precision == LSSPH_MIXED;
-  if (mixed) launch_fill_i32(s, a.fb_count, 0, 1);
   const int cls = d.opt.target_problem == 1 ? 0 : (d.opt.has_tem ? 2 : 1);
   const bool delta = d.opt.delta != 0;
   const int kt = d.opt.kernel_type;
+  const int model = d.opt.sph_model;
+  if (model <= 2) {  // classical SPH, FP64
+    const bool sum = model == 1;
+    if (cls == 0)
+      csph_launch<0, false>(s, a, d.block, lds, sum);
+    else if (cls == 1)
+      delta ? csph_launch<1, true>(s, a, d.block, lds, sum) : csph_launch<1, false>(s, a, d.block, lds, sum);
+    else
+      delta ? csph_launch<2, true>(s, a, d.block, lds, sum) : csph_launch<2, false>(s, a, d.block, lds, sum);
+    return;
+  }
+  if (model >= 4) {  // q = 9, 14, 20: one wavefront per particle, FP64
+    launch_lsa_wave(s, d, a, model, cls);
+    return;
+  }
+  const bool mixed = d.opt.precision == LSSPH_MIXED;
+  if (mixed) launch_fill_i32(s, a.fb_count, 0, 1);
   if (cls == 0)
     lsa_dispatch<0, false>(s, a, d.block, lds, mixed, kt);
   else if (cls == 1)
@@ -731,9 +439,9 @@ template <bool T, bool MIXED>
 static void pst_launch(hipStream_t s, const PstArgs &a, int block, size_t lds, int kt) {
   const dim3 g(xcd_grid(a.n, block)), b(block);
   if (kt == 3)
-    hipLaunchKernelGGL((k_pst<T, MIXED, 3>), g, b, lds, s, a);
+    hipLaunchKernelGGL((k_pst<T, MIXED, 3, 5>), g, b, lds, s, a);
   else
-    hipLaunchKernelGGL((k_pst<T, MIXED, 0>), g, b, lds, s, a);
+    hipLaunchKernelGGL((k_pst<T, MIXED, 0, 5>), g, b, lds, s, a);
 }
 
 void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
@@ -742,6 +450,7 @@ void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
   a.mout = d.mir[cur ^ 1];
   a.sinfo = d.sinfo;
   a.items = d.items;
+  a.orig_of = d.orig_of;
   a.cstart = d.cell_start;
   a.inst = d.inst;
   a.umax = d.umax;
@@ -751,8 +460,21 @@ void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;
   const size_t lds = list_lds_bytes(d.list_len, d.block);
-  const bool mixed = d.opt.precision == LSSPH_MIXED;
   const int kt = d.opt.kernel_type;
+  const int model = d.opt.sph_model;
+  if (model <= 2) {  // q = 2 fit of the classical models, FP64
+    const dim3 g(xcd_grid(a.n, d.block)), b(d.block);
+    if (d.opt.has_tem)
+      hipLaunchKernelGGL((k_pst<true, false, 0, 2>), g, b, lds, s, a);
+    else
+      hipLaunchKernelGGL((k_pst<false, false, 0, 2>), g, b, lds, s, a);
+    return;
+  }
+  if (model >= 4) {
+    launch_pst_wave(s, d, a, model);
+    return;
+  }
+  const bool mixed = d.opt.precision == LSSPH_MIXED;
   if (d.opt.has_tem)
     mixed ? pst_launch<true, true>(s, a, d.block, lds, kt) : pst_launch<true, false>(s, a, d.block, lds, kt);
   else

@@ -284,6 +284,22 @@ __device__ __forceinline__ int ref_cell_index(const InstDev &I, double x, double
   return cx + (cy - 1) * I.rnx;
 }
 
+// Reference cell coordinates (0-based) of a position: see ref_cell_index.
+__device__ __forceinline__ void ref_cell_xy(const InstDev &I, double x, double y, bool wall, int &cx, int &cy) {
+  if (I.periodic) {
+    cx = (int)floor(x * I.inv_rcw_x);
+    cy = (int)floor(y * I.inv_rcw_y);
+    cx = (cx == I.rnx) ? I.rnx - 1 : cx;
+    cy = (cy == I.rny) ? I.rny - 1 : cy;
+  } else if (wall) {
+    cx = (int)ceil(x / I.h_eff) - 1;
+    cy = (int)ceil(y / I.h_eff) - 1;
+  } else {
+    cx = (int)ceil(x * I.inv_h_eff) - 1;
+    cy = (int)ceil(y * I.inv_h_eff) - 1;
+  }
+}
+
 // Bookkeeping of a moved particle at slot s between cell builds.
 //   guard:  the particle may move at most skin / 2 from where the current
 //           cell list was built (xbuild); beyond that the 3x3 stencil could

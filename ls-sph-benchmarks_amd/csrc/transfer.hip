@@ -28,22 +28,6 @@ __global__ void k_nbr_count(int n, const Rec *__restrict__ mir, const int32_t *_
   counts_orig[orig_of[items[s]]] = cnt;
 }
 
-// Reference cell coordinates (0-based) of a position: see ref_cell_index.
-__device__ __forceinline__ void ref_cell_xy(const InstDev &I, double x, double y, bool wall, int &cx, int &cy) {
-  if (I.periodic) {
-    cx = (int)floor(x * I.inv_rcw_x);
-    cy = (int)floor(y * I.inv_rcw_y);
-    cx = (cx == I.rnx) ? I.rnx - 1 : cx;
-    cy = (cy == I.rny) ? I.rny - 1 : cy;
-  } else if (wall) {
-    cx = (int)ceil(x / I.h_eff) - 1;
-    cy = (int)ceil(y / I.h_eff) - 1;
-  } else {
-    cx = (int)ceil(x * I.inv_h_eff) - 1;
-    cy = (int)ceil(y * I.inv_h_eff) - 1;
-  }
-}
-
 // The neighbors of each fluid particle in the reference's summation order:
 // by position of the neighbor's reference cell in the 3x3 stencil of the
 // particle's reference cell (rows dy = -1..1, columns dx = -1..1), then by

@@ -250,12 +250,12 @@ contains
     end subroutine fatal
 
     ! The conditions of check_param_mod.f90:22-284, plus the options of the
-    ! GPU path (SPH_MODEL 3, RK 2/4, PRECISION 0/1).
+    ! GPU path (RK 2/4, PRECISION 0/1).
     subroutine check_param(param)
         type(Param_type), intent(in) :: param
         if (param%target_problem < 1 .or. param%target_problem > 4) call fatal("TARGET_PROBLEM must be between 1 and 4.")
         if (param%kernel_type < 1 .or. param%kernel_type > 5) call fatal("KERNEL_TYPE must be between 1 and 5.")
-        if (param%sph_model /= 3) call fatal("SPH_MODEL must be 3 (LS-SPH, 2nd-order) on the GPU path.")
+        if (param%sph_model < 1 .or. param%sph_model > 6) call fatal("SPH_MODEL must be between 1 and 6.")
         if (param%wall_model < 1 .or. param%wall_model > 3) call fatal("WALL_MODEL must be between 1 and 3.")
         if (param%rk /= 2 .and. param%rk /= 4) call fatal("RK must be 2 or 4.")
         if (param%precision /= 0 .and. param%precision /= 1) call fatal("PRECISION must be 0 or 1.")

@@ -17,10 +17,11 @@ FIELDS = ("x", "y", "u", "v", "rho", "pre")
 
 # Tolerance of the FP64 comparisons per case.  1e-12 everywhere except q = 20
 # (SPH_MODEL 6): its 20 x 20 moment matrices have condition numbers of 7e6 to
-# 4e7 on these lattices (2-norm, numpy), which turns the different operation
-# order of MKL's blocked LU and the textbook LU (GESV with partial pivoting)
-# into differences of up to 1e-11 in RHS_rho (bound cond * eps ~ 5e-9).
-CASE_TOL = {"tg_ls20_rk2": 1e-10}
+# 4e7 on these lattices (2-norm, numpy), so last-bit differences in the sums
+# (FMA contraction, MKL's blocked LU against the textbook LU) grow by up to
+# cond * eps ~ 5e-9.  Measured: oracle vs reference 9e-12 (first RHS),
+# HIP vs reference 1.1e-10 (after 4 steps).
+CASE_TOL = {"tg_ls20_rk2": 1e-9}
 
 
 def case_tol(case: str, default: float = 1e-12) -> float:

@@ -44,7 +44,7 @@ def _params(**kw):
     return p
 
 
-@pytest.mark.parametrize("field,value", [("target_problem", 7), ("kernel_type", 0), ("sph_model", 5),
+@pytest.mark.parametrize("field,value", [("target_problem", 7), ("kernel_type", 0), ("sph_model", 7),
                                          ("wall_model", 4), ("rk", 3), ("precision", 2), ("h", -1.0)])
 def test_create_rejects_bad_options_without_touching_the_gpu(field, value):
     lib = engine.load_library()
