@@ -181,6 +181,34 @@ def field_diff(a: dict, b: dict, n: int) -> dict:
     return out
 
 
+MODEL_NAMES = {1: "classical SPH (pressure sum model)", 2: "classical SPH (difference model)",
+               4: "LS q=9 (3rd-order Taylor)", 5: "LS q=14 (4th-order Taylor)", 6: "LS q=20 (5th-order Taylor)"}
+
+
+def model_bench(args) -> None:
+    """The same periodic Taylor-Green workload (RK4 + PST every step) with one
+    of the other operators, FP64; a line of its own, not the headline."""
+    from lssph import config as lcfg
+    n = args.n
+    base, st = workloads.taylor_green_periodic(n=n, re=1000.0, rk=4, jitter=0.2, seed=2029)
+    param = lcfg.init_param(base.cfg.replace(sph_model=args.sph_model))
+    r = run_gpu(param, st, 0, args.steps, args.warmup)
+    kt = r["kt"]
+    n_part = st.num_int
+    print(json.dumps({
+        "metric": f"particle-steps/s, SPH_MODEL {args.sph_model}", "value": round(n_part * args.steps / (r["ms"] / 1e3), 1),
+        "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(r["ms"] / args.steps, 4), "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"periodic Taylor-Green {n}x{n}, jitter 0.2dx, Re 1000, h_eff 3.1dx, "
+                               f"{MODEL_NAMES[args.sph_model]}, RK4 + PST every step",
+                   "particles": n_part},
+        "ms_per_step_by_kernel": {k: round(getattr(kt, k + "_ms") / args.steps, 4)
+                                  for k in ("lsa", "pst", "cells", "ghost", "other")},
+        "operator_ms_per_launch": {"rhs": round(kt.lsa_ms / max(kt.lsa_launches, 1), 4),
+                                   "pst": round(kt.pst_ms / max(kt.pst_launches, 1), 4)},
+        "tg_l2_error": r["l2"]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +220,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (calibrates rocprof FETCH/WRITE_SIZE)")
+    ap.add_argument("--sph-model", type=int, default=3,
+                    help="1-2 classical SPH, 4-6 LS q=9/14/20: time that operator (FP64) instead of the headline")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -201,6 +231,9 @@ def main():
               "(a domain decomposition is designed but not built); run with --gpus 1.")
         sys.exit(2)
 
+    if args.sph_model != 3:
+        model_bench(args)
+        return
     n = args.n
     param, st = workloads.taylor_green_periodic(n=n, re=1000.0, rk=4, jitter=0.2, seed=2029)
     n_part = st.num_int
