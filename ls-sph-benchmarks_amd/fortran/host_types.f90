@@ -42,7 +42,7 @@ module global_types
         double precision   :: dt_CFL_relax, dt_vis_relax
         ! options of this host
         integer            :: rk, periodic, precision
-        double precision   :: diff_coef, diff_source, cond_max
+        double precision   :: diff_coef, diff_source, cond_max, cell_skin
     end type Param_type
 
     ! SP_type of global_types.f90:78-103: host mirror of the device state.
@@ -175,6 +175,7 @@ contains
         param%diff_coef      = DIFF_COEF
         param%diff_source    = DIFF_SOURCE
         param%cond_max       = COND_MAX
+        param%cell_skin      = CELL_SKIN
 
         if (trim(adjustl(param%read_name)) == "new") then
             param%total_step = param%end_step

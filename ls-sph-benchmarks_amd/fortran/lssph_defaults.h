@@ -24,6 +24,9 @@
 #ifndef HEAT_MODE
 #define HEAT_MODE 0         /* 1: diffusion start f = sin 2pi x sin 2pi y */
 #endif
+#ifndef CELL_SKIN
+#define CELL_SKIN 0.08d0    /* cell-list skin / h_eff; 0: a cell build after every pass */
+#endif
 #ifndef USE_GRAPH
 #define USE_GRAPH 1         /* 1: steps replayed from a captured HIP graph */
 #endif
