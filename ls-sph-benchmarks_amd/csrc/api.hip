@@ -21,6 +21,7 @@ struct lssph_handle {
   std::vector<int32_t> inst_of;  // upload index -> instance
   int cur = 0;          // records buffer holding the current state
   int passes = 0;       // mover passes since the last cell build
+  bool spd_fresh = false;  // Device::spd holds the speeds of the current slots
   int max_passes = 1;   // passes a cell list serves (rk + 1 with a skin, else 1)
   double skin = 0.08;   // cell-grid skin as a fraction of h_eff
   bool uploaded = false, cells_ready = false;
@@ -273,21 +274,24 @@ void timed(lssph_handle *h, int group, F &&f) {
 // r^2 <= h_eff^2 at the current positions.  The pass that uses a list up
 // also writes the new cell keys, and the incremental build follows it.  The
 // shift (up to 0.2 h per particle) always ends a list.
+// Returns whether a cell build followed the pass.
 template <typename F>
-void mover_pass(lssph_handle *h, int group, bool ends_list, F &&launch) {
+bool mover_pass(lssph_handle *h, int group, bool ends_list, F &&launch) {
   Device &d = h->d;
   const bool moving = !is_diffusion(h);
   const bool build = moving && (ends_list || h->passes + 1 >= h->max_passes);
   const int guard = (moving && !build && h->max_passes > 1) ? 1 : 0;
   timed(h, group, [&] { launch(guard, build ? 1 : 0); });
   h->cur ^= 1;
-  if (!moving) return;
+  h->spd_fresh = false;
+  if (!moving) return false;
   ++h->passes;
   if (build) {
     timed(h, kGrpCells, [&] { launch_rebuild(h->stream, d, h->cur, h->cur ^ 1); });
     h->cur ^= 1;
     h->passes = 0;
   }
+  return build;
 }
 
 // One RK stage pass: fused operator + update, the cell list, and ghost on
@@ -295,7 +299,12 @@ void mover_pass(lssph_handle *h, int group, bool ends_list, F &&launch) {
 // test's particles do not move (RK4_mod.f90:295-297).
 void enqueue_stage(lssph_handle *h, StageMode m) {
   Device &d = h->d;
-  mover_pass(h, kGrpLsa, false, [&](int guard, int keys) { launch_lsa(h->stream, d, h->cur, m, guard, keys); });
+  // the last stage of a step that shifts stores the speeds U_max needs
+  const bool spd = (m == kRk2S2 || m == kRk4S4) && !is_diffusion(h);
+  const bool built = mover_pass(h, kGrpLsa, false, [&](int guard, int keys) {
+    launch_lsa(h->stream, d, h->cur, m, guard, keys, spd);
+  });
+  h->spd_fresh = spd && !built;  // a build moves the slots
   if (d.n_ext) timed(h, kGrpGhost, [&] { launch_ghost(h->stream, d, h->cur); });
 }
 
@@ -314,7 +323,12 @@ void enqueue_rk(lssph_handle *h) {
 // PST_mod.f90:37-114
 void enqueue_pst(lssph_handle *h) {
   Device &d = h->d;
-  timed(h, kGrpOther, [&] { launch_umax(h->stream, d, h->cur); });
+  timed(h, kGrpOther, [&] {
+    if (h->spd_fresh)
+      launch_umax_spd(h->stream, d);
+    else
+      launch_umax(h->stream, d, h->cur);
+  });
   mover_pass(h, kGrpPst, true, [&](int guard, int keys) { launch_pst(h->stream, d, h->cur, guard, keys); });
   if (d.n_ext) timed(h, kGrpGhost, [&] { launch_ghost(h->stream, d, h->cur); });
 }
@@ -344,6 +358,7 @@ int reorder(lssph_handle *h) {
   launch_iota(s, d.slot_of, n);
   launch_vm_init(s, d, h->cur);
   h->steps_since_reorder = 0;
+  h->spd_fresh = false;
   return 0;
 }
 
@@ -352,6 +367,7 @@ void full_build(lssph_handle *h, int fixed_key_mode) {
   launch_bin(h->stream, h->d, h->cur, h->cur ^ 1, fixed_key_mode);
   h->cur ^= 1;
   h->passes = 0;
+  h->spd_fresh = false;
 }
 
 void drop_graphs(lssph_handle *h) {
@@ -492,6 +508,7 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   for (int f = 0; f < 6 && !rc; ++f) rc = dalloc(h, &d.new_[f], n);
   for (int f = 0; f < 4 && !rc; ++f) rc = dalloc(h, &d.rhs[f], n);
   rc = rc ? rc : dalloc(h, &d.f_prev, n);
+  rc = rc ? rc : dalloc(h, &d.spd, n);
   const size_t m = d.n_ext;
   rc = rc ? rc : dalloc(h, &d.vm_x, m);
   rc = rc ? rc : dalloc(h, &d.vm_y, m);

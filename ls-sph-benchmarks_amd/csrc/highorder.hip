@@ -374,6 +374,7 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_lsa_wave(LsaArgs A) {
   const int32_t inf = A.sinfo[s];
   if (info_ptype(inf) != 0) {
     if (lane == 0 && A.mode != kRhsOnly) A.mout[s] = A.mir[s];
+    if (lane == 0 && A.spd) A.spd[s] = 0.0;
     return;
   }
   const InstDev &I = A.inst[info_inst(inf)];

@@ -65,6 +65,7 @@ struct Device {
   double *new_[6] = {};
   double *rhs[4] = {};   // u, v, rho, tem (lssph_calculate_rhs only)
   double *f_prev = nullptr;  // diffusion steady check
+  double *spd = nullptr;     // per slot: |u| after the last RK stage (0 for walls), for U_max
   // virtual markers (marker k pairs with upload index n_int + k)
   double *vm_x = nullptr, *vm_y = nullptr, *vm_nx = nullptr, *vm_ny = nullptr;
   double *vm_u = nullptr, *vm_v = nullptr, *vm_pre = nullptr, *vm_tem = nullptr;
@@ -105,7 +106,10 @@ void launch_inverse_perm(hipStream_t s, int n, const int32_t *orig_of, int32_t *
 enum StageMode { kRhsOnly = 0, kRk2S1, kRk2S2, kRk4S1, kRk4S2, kRk4S3, kRk4S4 };
 // Mover passes read records buffer cur and write buffer cur ^ 1; guard
 // checks each move against the skin, keys prepares the next cell build.
-void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, int keys);
+// write_spd: the last RK stage also stores each particle's speed for U_max.
+void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, int keys, bool write_spd = false);
+// U_max from the speeds the last RK stage stored (valid while slots are unchanged).
+void launch_umax_spd(hipStream_t s, Device &d);
 void launch_umax(hipStream_t s, Device &d, int cur);
 void launch_dfmax(hipStream_t s, Device &d, int cur);
 void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys);

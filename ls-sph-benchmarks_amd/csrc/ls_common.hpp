@@ -197,6 +197,7 @@ struct LsaArgs {
   double *old_[6];
   double *new_[6];
   double *rhs[4];
+  double *spd;  // non-null in the last RK stage: |u| of the new state per slot
   unsigned long long *stats64;
   int32_t *counters;
   int32_t *fb_list;
@@ -399,6 +400,7 @@ __device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p
   if (!DIFF) {
     out.pre = I.c_sq * (out.rho - I.rho_ref);  // equation of state
     out.vol = I.sp_mass / out.rho;
+    if (A.spd) A.spd[s] = sqrt(out.u * out.u + out.v * out.v);  // PST_mod.f90:51
   }
   A.mout[s] = out;
   if (!DIFF) record_move(A.mv, I, s, out.x, out.y);

@@ -31,6 +31,7 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   const int32_t inf = A.sinfo[s];
   if (info_ptype(inf) != 0) {
     if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];  // walls move on with the records
+    if (A.spd) A.spd[s] = 0.0;
     return;
   }
   const InstDev &I = A.inst[info_inst(inf)];
@@ -102,6 +103,7 @@ __global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
   const int32_t inf = A.sinfo[s];
   if (info_ptype(inf) != 0) {
     if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];
+    if (A.spd) A.spd[s] = 0.0;
     return;
   }
   const InstDev &I = A.inst[info_inst(inf)];
@@ -317,6 +319,19 @@ __global__ void __launch_bounds__(256) k_umax(const Rec *__restrict__ rec, const
   if (threadIdx.x == 0) atomic_max_nonneg(&umax[blockIdx.y], m);
 }
 
+// The same maximum over the speeds the last RK stage stored (0 for walls):
+// 8 B per slot instead of the records' u, v.
+__global__ void __launch_bounds__(256) k_umax_spd(const double *__restrict__ spd, const int32_t *__restrict__ cstart,
+                                                  const InstDev *__restrict__ inst, unsigned long long *umax) {
+  const InstDev &I = inst[blockIdx.y];
+  const int t0 = cstart[I.cell_off], t1 = cstart[I.cell_off + I.nx * I.ny];
+  double s = 0.0;
+  for (int t = t0 + blockIdx.x * blockDim.x + threadIdx.x; t < t1; t += gridDim.x * blockDim.x)
+    s = fmax(s, spd[t]);
+  const double m = block_max(s);
+  if (threadIdx.x == 0) atomic_max_nonneg(&umax[blockIdx.y], m);
+}
+
 // max|f_previous - u| of check_steady_state_mod.f90:36-46 (f_previous in
 // physical order).
 __global__ void __launch_bounds__(256) k_dfmax(const Rec *__restrict__ rec, const int32_t *__restrict__ sinfo,
@@ -371,8 +386,9 @@ static void lsa_dispatch(hipStream_t s, const LsaArgs &a, int block, size_t lds,
   }
 }
 
-void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, int keys) {
+void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, int keys, bool write_spd) {
   LsaArgs a;
+  a.spd = write_spd ? d.spd : nullptr;
   a.mir = d.mir[cur];
   a.mout = d.mir[cur ^ 1];
   a.sinfo = d.sinfo;
@@ -427,6 +443,12 @@ void launch_umax(hipStream_t s, Device &d, int cur) {
   launch_fill_u64(s, d.umax, 0ull, d.n_inst);
   hipLaunchKernelGGL(k_umax, dim3(d.red_blocks, d.n_inst), dim3(256), 0, s, d.mir[cur], d.sinfo, d.cell_start,
                      d.inst, d.umax);
+}
+
+void launch_umax_spd(hipStream_t s, Device &d) {
+  launch_fill_u64(s, d.umax, 0ull, d.n_inst);
+  hipLaunchKernelGGL(k_umax_spd, dim3(d.red_blocks, d.n_inst), dim3(256), 0, s, d.spd, d.cell_start, d.inst,
+                     d.umax);
 }
 
 void launch_dfmax(hipStream_t s, Device &d, int cur) {
