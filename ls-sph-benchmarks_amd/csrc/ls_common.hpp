@@ -63,6 +63,30 @@ struct LsSums {
       for (int r = 0; r <= i; ++r) M[i * (i + 1) / 2 + r] += vwa * a[r];
     }
   }
+  // Mixed path, q = 5: with the basis [x, y, x^2/2, y^2/2, xy] the moment
+  // matrix has 12 distinct sums, M(2,1) = M(4,0)/2, M(3,0) = M(4,1)/2 and
+  // M(3,2) = M(4,4)/4, so add_reduced skips those three and complete()
+  // derives them after the loop (exact scalings; the results differ from
+  // direct accumulation by FP32 rounding of the sums only).
+  __device__ __forceinline__ void add_reduced(R vw, const R (&a)[Q], const R (&df)[NF]) {
+    static_assert(Q == 5, "q = 5 basis");
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const R vwa = vw * a[i];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) b[f][i] += vwa * df[f];
+#pragma unroll
+      for (int r = 0; r <= i; ++r) {
+        if ((i == 2 && r == 1) || (i == 3 && r == 0) || (i == 3 && r == 2)) continue;
+        M[i * (i + 1) / 2 + r] += vwa * a[r];
+      }
+    }
+  }
+  __device__ __forceinline__ void complete() {
+    M[4] = R(0.5) * M[10];   // (2,1) from (4,0)
+    M[6] = R(0.5) * M[11];   // (3,0) from (4,1)
+    M[8] = R(0.25) * M[14];  // (3,2) from (4,4)
+  }
 };
 
 // Taylor basis a = [x/h, y/h, x^2/2h^2, y^2/2h^2, xy/h^2] (LSSPH_A_mod.f90:
@@ -257,13 +281,15 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
                  [&](const Rec &nb, double dx, double dy) {
                    const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
                    const float q = fast_sqrtf(xs * xs + ys * ys);
-                   const float vw = float(nb.vol * I.w_norm) * kshape_w<float, KT>(kt, q);
+                   // the common factor w_norm of M and b cancels in the solve
+                   const float vw = float(nb.vol) * kshape_w<float, KT>(kt, q);
                    float a[5];
                    basis5s(xs, ys, a);
                    float df[NF];
                    field_diffs<CLS, DELTA>(nb, me, df);
-                   S.add(vw, a, df);
+                   S.add_reduced(vw, a, df);
                  });
+  S.complete();
   double X[NF][5];
   if (!ldlt_solve_refine<NF>(S, X, I.cond_max)) return false;
   scale_derivs<NF>(X, I, d);

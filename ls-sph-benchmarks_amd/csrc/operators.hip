@@ -218,20 +218,33 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
                    } else {
                      const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
                      const float q = fast_sqrtf(xs * xs + ys * ys);
-                     const float shp = kshape_w<float, KT>(kt, q);
-                     const float vwn = float(nb.vol * I.w_norm);
-                     // dW/dr * (-x/r) * V = V * w_norm * dw_fac * shape'(q) * (-xs / q)
-                     const float g0 = float(nb.vol * I.w_norm * I.dw_fac) * kshape_dw<float, KT>(kt, q) * fast_rcpf(q);
-                     const float g = q > 0.0f ? g0 : 0.0f;  // the particle itself adds no shift
+                     const float vol = float(nb.vol);
+                     float shp, g;
+                     if constexpr (KT == 3) {
+                       // Wendland C2: shape = t^4 (1 + 2q), shape' = -5 q t^3 with
+                       // t = 1 - q/2, so dW/dr * (-x/r) * V = V w_norm dw_fac
+                       // (-5 t^3) (-xs): no division by q, and the particle
+                       // itself (xs = ys = 0) adds nothing
+                       const float t = 1.0f - 0.5f * q, t3 = t * t * t;
+                       shp = q > 2.0f ? 0.0f : t3 * t * (1.0f + 2.0f * q);
+                       g = vol * float(I.w_norm * I.dw_fac) * (-5.0f * t3);
+                     } else {
+                       shp = kshape_w<float, KT>(kt, q);
+                       // dW/dr * (-x/r) * V = V * w_norm * dw_fac * shape'(q) * (-xs / q)
+                       const float g0 = vol * float(I.w_norm * I.dw_fac) * kshape_dw<float, KT>(kt, q) * fast_rcpf(q);
+                       g = q > 0.0f ? g0 : 0.0f;  // the particle itself adds no shift
+                     }
                      const float wr = float(I.w_norm * inv_w_ave) * shp;
                      const float f = 1.0f + 0.2f * ((wr * wr) * (wr * wr));
                      sx = sx + f * (g * -xs);
                      sy = sy + f * (g * -ys);
                      float a[5];
                      basis5s(xs, ys, a);
-                     S.add(vwn * shp, a, df);
+                     // the common factor w_norm of M and b cancels in the solve
+                     S.add_reduced(vol * shp, a, df);
                    }
                  });
+  if constexpr (MIXED) S.complete();
   // shift vector and its clamp (shift_interpolate_mod.f90:333-346)
   const double umax = __longlong_as_double((long long)A.umax[ii]);
   const double coe = I.pst_c * umax * I.dt * I.h;
