@@ -136,6 +136,7 @@ InstDev make_inst(const lssph_params_t &p, double skin) {
   I.inv_h = 1.0 / p.h;
   I.inv_h2 = I.inv_h / p.h;
   I.inv_h_sq = I.inv_h * I.inv_h;
+  I.inv_h3 = I.inv_h2 / p.h;
   I.h_eff = p.h_eff;
   I.h_eff_sq = p.h_eff * p.h_eff;
   I.inv_h_eff = 1.0 / p.h_eff;

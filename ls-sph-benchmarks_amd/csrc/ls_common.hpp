@@ -100,15 +100,26 @@ __device__ __forceinline__ void basis5(double x, double y, double ih, double ih2
 }
 
 // The q-term basis: q = 2 keeps the first-order terms (the shifting of the
-// classical models), q = 5 is basis5.
+// classical models), q = 5 is basis5, q = 9 adds the 3rd-order terms
+// (LSSPH_A_mod.f90:148-153, shift_interpolate_mod.f90:159-164).
 template <int Q>
-__device__ __forceinline__ void basis_q(double x, double y, double ih, double ih2, double (&a)[Q]) {
-  static_assert(Q == 2 || Q == 5, "q = 2 or 5");
-  if constexpr (Q == 5) {
-    basis5(x, y, ih, ih2, a);
+__device__ __forceinline__ void basis_q(double x, double y, const InstDev &I, double (&a)[Q]) {
+  static_assert(Q == 2 || Q == 5 || Q == 9, "q = 2, 5 or 9");
+  if constexpr (Q == 2) {
+    a[0] = x * I.inv_h;
+    a[1] = y * I.inv_h;
   } else {
-    a[0] = x * ih;
-    a[1] = y * ih;
+    a[0] = x * I.inv_h;
+    a[1] = y * I.inv_h;
+    a[2] = 0.5 * (x * x) * I.inv_h2;
+    a[3] = 0.5 * (y * y) * I.inv_h2;
+    a[4] = x * y * I.inv_h2;
+  }
+  if constexpr (Q == 9) {
+    a[5] = 1.0 / 6.0 * (x * x * x) * I.inv_h3;
+    a[6] = 1.0 / 6.0 * (y * y * y) * I.inv_h3;
+    a[7] = 0.5 * (x * x) * y * I.inv_h3;
+    a[8] = 0.5 * (y * y) * x * I.inv_h3;
   }
 }
 
@@ -244,28 +255,34 @@ __device__ __forceinline__ void scale_derivs(const double (&X)[NF][5], const Ins
 
 // FP64 assembly + DGESV-semantics solve; d[f] = (d_x, d_y, d_xx, d_yy, d_xy)
 // (LSSPH_A_mod.f90:199-240).
-template <int CLS, bool DELTA, int KT>
+// q = 5, or 9 (SPH_MODEL 4, whose 45 + 9 F sums still fit one thread).
+template <int CLS, bool DELTA, int KT, int Q = 5>
 __device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
                                          const int32_t *lds, int cnt, int list_len, int B, int tid,
                                          double (&d)[Layout<CLS, DELTA>::NF][5]) {
   constexpr int NF = Layout<CLS, DELTA>::NF;
   const int kt = A.kt;
-  LsSums<double, NF> S;
+  LsSums<double, NF, Q> S;
   S.zero();
   walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, list_len, B, tid,
                  [&](const Rec &nb, double dx, double dy) {
                    const double r = sqrt(sq_dist(dx, dy));
                    const double w = I.w_norm * kshape_w<double, KT>(kt, r * I.inv_h);
-                   double a[5];
-                   basis5(dx, dy, I.inv_h, I.inv_h2, a);
+                   double a[Q];
+                   basis_q<Q>(dx, dy, I, a);
                    double df[NF];
                    field_diffs<CLS, DELTA>(nb, me, df);
                    S.add(nb.vol * w, a, df);
                  });
-  double M[5][5], X[NF][5];
+  double M[Q][Q], X[NF][Q];
   unpack(S, M, X);
-  lu_solve<5, NF>(M, X);
-  scale_derivs<NF>(X, I, d);
+  lu_solve<Q, NF>(M, X);
+  double X5[NF][5];
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int i = 0; i < 5; ++i) X5[f][i] = X[f][i];
+  scale_derivs<NF>(X5, I, d);
 }
 
 // Mixed-precision assembly + FP64 LDL^T with refinement; false -> fallback.

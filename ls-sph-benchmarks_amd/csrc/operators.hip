@@ -21,8 +21,10 @@
 
 namespace lssph {
 
-template <int CLS, bool MIXED, bool DELTA, int KT>
+// Q = 5 (SPH_MODEL 3), or 9 (SPH_MODEL 4, FP64).
+template <int CLS, bool MIXED, bool DELTA, int KT, int Q = 5>
 __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
+  static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
   constexpr int NF = Layout<CLS, DELTA>::NF;
   extern __shared__ int32_t lds[];
   const int tid = threadIdx.x, B = blockDim.x;
@@ -48,7 +50,7 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   for (int f = 0; f < NF; ++f)
 #pragma unroll
     for (int i = 0; i < 5; ++i) d[f][i] = 0.0;
-  if (cnt - 1 >= 5) {  // LSSPH_A_mod.f90:208 (num_neighbors >= q)
+  if (cnt - 1 >= Q) {  // LSSPH_A_mod.f90:208 (num_neighbors >= q)
     if constexpr (MIXED) {
       if (!derivs32<CLS, DELTA, KT>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d)) {
         const int k = atomicAdd(A.fb_count, 1);
@@ -56,7 +58,7 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
         return;
       }
     } else {
-      derivs64<CLS, DELTA, KT>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d);
+      derivs64<CLS, DELTA, KT, Q>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d);
     }
   }
   // Re-read the particle's own record for the epilogue instead of keeping it
@@ -213,7 +215,7 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
                      sx = sx + f * dwx;
                      sy = sy + f * dwy;
                      double a[Q];
-                     basis_q<Q>(dx, dy, I.inv_h, I.inv_h2, a);
+                     basis_q<Q>(dx, dy, I, a);
                      S.add(vj * w, a, df);
                    } else {
                      const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
@@ -274,7 +276,7 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
       for (int i = 0; i < Q; ++i) X[f][i] = 0.0;
   }
   double a[Q];
-  basis_q<Q>(dxs, dys, I.inv_h, I.inv_h2, a);
+  basis_q<Q>(dxs, dys, I, a);
   double acc[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -438,7 +440,21 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
       delta ? csph_launch<2, true>(s, a, d.block, lds, sum) : csph_launch<2, false>(s, a, d.block, lds, sum);
     return;
   }
-  if (model >= 4) {  // q = 9, 14, 20: one wavefront per particle, FP64
+  if (model == 4) {  // q = 9, FP64, one thread per particle
+    const dim3 g(xcd_grid(a.n, d.block)), b(d.block);
+    if (cls == 0)
+      hipLaunchKernelGGL((k_lsa<0, false, false, 0, 9>), g, b, lds, s, a);
+    else if (cls == 1 && delta)
+      hipLaunchKernelGGL((k_lsa<1, false, true, 0, 9>), g, b, lds, s, a);
+    else if (cls == 1)
+      hipLaunchKernelGGL((k_lsa<1, false, false, 0, 9>), g, b, lds, s, a);
+    else if (delta)
+      hipLaunchKernelGGL((k_lsa<2, false, true, 0, 9>), g, b, lds, s, a);
+    else
+      hipLaunchKernelGGL((k_lsa<2, false, false, 0, 9>), g, b, lds, s, a);
+    return;
+  }
+  if (model >= 5) {  // q = 14, 20: one wavefront per particle, FP64
     launch_lsa_wave(s, d, a, model, cls);
     return;
   }
@@ -505,7 +521,15 @@ void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
       hipLaunchKernelGGL((k_pst<false, false, 0, 2>), g, b, lds, s, a);
     return;
   }
-  if (model >= 4) {
+  if (model == 4) {  // q = 9, FP64, one thread per particle
+    const dim3 g(xcd_grid(a.n, d.block)), b(d.block);
+    if (d.opt.has_tem)
+      hipLaunchKernelGGL((k_pst<true, false, 0, 9>), g, b, lds, s, a);
+    else
+      hipLaunchKernelGGL((k_pst<false, false, 0, 9>), g, b, lds, s, a);
+    return;
+  }
+  if (model >= 5) {
     launch_pst_wave(s, d, a, model);
     return;
   }
