@@ -137,6 +137,7 @@ InstDev make_inst(const lssph_params_t &p, double skin) {
   I.inv_h2 = I.inv_h / p.h;
   I.inv_h_sq = I.inv_h * I.inv_h;
   I.inv_h3 = I.inv_h2 / p.h;
+  I.inv_h4 = I.inv_h3 / p.h;
   I.h_eff = p.h_eff;
   I.h_eff_sq = p.h_eff * p.h_eff;
   I.inv_h_eff = 1.0 / p.h_eff;
@@ -219,7 +220,7 @@ int check_counters(lssph_handle *h) {
   if (c[kWaveListOverflow] > 0) {
     char buf[200];
     std::snprintf(buf, sizeof buf,
-                  "%d particle(s) have more than %d neighbors, the limit of the SPH_MODEL 4-6 kernels",
+                  "%d particle(s) have more than %d neighbors, the limit of the SPH_MODEL 6 kernels",
                   c[kWaveListOverflow], kWaveList);
     return fail(h, LSSPH_E_CAPACITY, buf);
   }

@@ -34,6 +34,7 @@ struct InstDev {
   double h, inv_h, inv_h2, h_eff_sq, inv_h_eff, h_eff;
   double inv_h_sq;    // inv_h * inv_h (type-B basis, interpolate_LSSPH_B_mod.f90:152)
   double inv_h3;      // inv_h2 / h (3rd-order Taylor terms, LSSPH_A_mod.f90:148-153)
+  double inv_h4;      // inv_h3 / h (4th-order terms, :155-162)
   double threshold;   // steady-state threshold
   double sp_mass, w_ave, wl_thick, len_x, len_y;
   double inv_cw_x, inv_cw_y;  // internal cell grid: 1 / cell width (>= h_eff + skin)

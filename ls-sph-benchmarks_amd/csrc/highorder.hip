@@ -1,10 +1,9 @@
-// highorder.hip -- LS type A with the 3rd- to 5th-order Taylor bases
-// (SPH_MODEL 4-6: q = 9, 14, 20; LSSPH_A_mod.f90:46-53, 148-172) and the
-// particle shifting with the same bases (shift_interpolate_mod.f90:45-52,
-// 153-185, 255-298), FP64.
+// highorder.hip -- LS type A with the 5th-order Taylor basis (SPH_MODEL 6:
+// q = 20; LSSPH_A_mod.f90:46-53, 148-172) and the particle shifting with the
+// same basis (shift_interpolate_mod.f90:45-52, 153-185, 255-298), FP64.
 //
-// A q x q moment matrix (q = 20: 210 sums plus 20 per field) does not fit
-// one thread's registers, so one wavefront handles one particle:
+// A 20 x 20 moment matrix (210 sums plus 20 per field) does not fit one
+// thread's registers, so one wavefront handles one particle:
 //   1. the lanes test the stencil's candidates 64 at a time and compact the
 //      accepted ones into an LDS list with a ballot;
 //   2. a rank sort puts the list into the reference's summation order
@@ -19,7 +18,8 @@
 //   5. lane 0 runs the epilogue of the q = 5 kernels (RHS, stage update,
 //      cell bookkeeping).
 // Waves never wait for each other (no block barrier), so a wave whose slot
-// is a wall or past the end simply returns.
+// is a wall or past the end simply returns.  (q = 9 and 14 run on the
+// per-thread kernels; the templates below also take them.)
 #include "ls_common.hpp"
 
 namespace lssph {
@@ -556,14 +556,9 @@ static void lsa_wave_q(hipStream_t s, const LsaArgs &a, int cls) {
     hipLaunchKernelGGL((k_lsa_wave<Q, 2>), g, b, 0, s, a);
 }
 
-void launch_lsa_wave(hipStream_t s, Device &, const LsaArgs &a, int sph_model, int cls) {
+void launch_lsa_wave(hipStream_t s, Device &, const LsaArgs &a, int, int cls) {
   if (a.n == 0) return;
-  if (sph_model == 4)
-    lsa_wave_q<9>(s, a, cls);
-  else if (sph_model == 5)
-    lsa_wave_q<14>(s, a, cls);
-  else
-    lsa_wave_q<20>(s, a, cls);
+  lsa_wave_q<20>(s, a, cls);
 }
 
 template <int Q>
@@ -577,13 +572,8 @@ static void pst_wave_q(hipStream_t s, const PstArgs &a, bool tem) {
 
 void launch_pst_wave(hipStream_t s, Device &d, const PstArgs &a, int sph_model) {
   if (a.n == 0) return;
-  const bool tem = d.opt.has_tem != 0;
-  if (sph_model == 4)
-    pst_wave_q<9>(s, a, tem);
-  else if (sph_model == 5)
-    pst_wave_q<14>(s, a, tem);
-  else
-    pst_wave_q<20>(s, a, tem);
+  (void)sph_model;
+  pst_wave_q<20>(s, a, d.opt.has_tem != 0);
 }
 
 }  // namespace lssph

@@ -17,7 +17,7 @@ enum {
   kFbCount = 3,
   kStrayOverflow = 4,
   kSkinExceeded = 5,
-  kWaveListOverflow = 6,  // a particle of the q = 9..20 kernels with too many neighbors
+  kWaveListOverflow = 6,  // a particle of the q = 20 kernels with too many neighbors
   kNumCounters = 7
 };
 
@@ -27,7 +27,7 @@ enum {
 constexpr int kStrayCap = 4096;
 constexpr int kArrivals = 8;  // arrivals per new cell merged directly
 // Neighbor-list capacity per particle of the one-wavefront-per-particle
-// kernels (SPH_MODEL 4-6).
+// kernels (SPH_MODEL 6).
 constexpr int kWaveList = 256;
 enum { kFallbacks = 0, kOverflows = 1, kNumStats64 = 2 };
 

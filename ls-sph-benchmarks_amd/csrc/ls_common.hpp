@@ -100,11 +100,12 @@ __device__ __forceinline__ void basis5(double x, double y, double ih, double ih2
 }
 
 // The q-term basis: q = 2 keeps the first-order terms (the shifting of the
-// classical models), q = 5 is basis5, q = 9 adds the 3rd-order terms
-// (LSSPH_A_mod.f90:148-153, shift_interpolate_mod.f90:159-164).
+// classical models), q = 5 is basis5, q = 9 and 14 add the 3rd- and
+// 4th-order terms (LSSPH_A_mod.f90:148-162, shift_interpolate_mod.f90:
+// 159-172).
 template <int Q>
 __device__ __forceinline__ void basis_q(double x, double y, const InstDev &I, double (&a)[Q]) {
-  static_assert(Q == 2 || Q == 5 || Q == 9, "q = 2, 5 or 9");
+  static_assert(Q == 2 || Q == 5 || Q == 9 || Q == 14, "q = 2, 5, 9 or 14");
   if constexpr (Q == 2) {
     a[0] = x * I.inv_h;
     a[1] = y * I.inv_h;
@@ -115,11 +116,18 @@ __device__ __forceinline__ void basis_q(double x, double y, const InstDev &I, do
     a[3] = 0.5 * (y * y) * I.inv_h2;
     a[4] = x * y * I.inv_h2;
   }
-  if constexpr (Q == 9) {
+  if constexpr (Q >= 9) {
     a[5] = 1.0 / 6.0 * (x * x * x) * I.inv_h3;
     a[6] = 1.0 / 6.0 * (y * y * y) * I.inv_h3;
     a[7] = 0.5 * (x * x) * y * I.inv_h3;
     a[8] = 0.5 * (y * y) * x * I.inv_h3;
+  }
+  if constexpr (Q >= 14) {
+    a[9] = 1.0 / 24.0 * (x * x * x * x) * I.inv_h4;
+    a[10] = 1.0 / 24.0 * (y * y * y * y) * I.inv_h4;
+    a[11] = 1.0 / 6.0 * (x * x * x) * y * I.inv_h4;
+    a[12] = 1.0 / 6.0 * (y * y * y) * x * I.inv_h4;
+    a[13] = 1.0 / 4.0 * (x * x) * (y * y) * I.inv_h4;
   }
 }
 
@@ -255,7 +263,7 @@ __device__ __forceinline__ void scale_derivs(const double (&X)[NF][5], const Ins
 
 // FP64 assembly + DGESV-semantics solve; d[f] = (d_x, d_y, d_xx, d_yy, d_xy)
 // (LSSPH_A_mod.f90:199-240).
-// q = 5, or 9 (SPH_MODEL 4, whose 45 + 9 F sums still fit one thread).
+// q = 5, 9 or 14 (SPH_MODEL 3-5).
 template <int CLS, bool DELTA, int KT, int Q = 5>
 __device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
                                          const int32_t *lds, int cnt, int list_len, int B, int tid,
@@ -464,8 +472,8 @@ struct PstArgs {
   int n, kt, list_len;
 };
 
-// The LS operators of SPH_MODEL 4-6 (q = 9, 14, 20), one wavefront per
-// particle (highorder.hip).  cls: 0 diffusion, 1 flow, 2 Boussinesq.
+// The LS operators of SPH_MODEL 6 (q = 20), one wavefront per particle
+// (highorder.hip).  cls: 0 diffusion, 1 flow, 2 Boussinesq.
 void launch_lsa_wave(hipStream_t s, Device &d, const LsaArgs &a, int sph_model, int cls);
 void launch_pst_wave(hipStream_t s, Device &d, const PstArgs &a, int sph_model);
 

@@ -21,7 +21,9 @@
 
 namespace lssph {
 
-// Q = 5 (SPH_MODEL 3), or 9 (SPH_MODEL 4, FP64).
+// Q = 5 (SPH_MODEL 3), 9 or 14 (SPH_MODEL 4-5, FP64; at q = 14 part of the
+// sums live in scratch, still several times faster than one wavefront per
+// particle).
 template <int CLS, bool MIXED, bool DELTA, int KT, int Q = 5>
 __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
@@ -165,7 +167,8 @@ __global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
 // Particle shifting (PST_mod.f90:37-114, shift_interpolate_mod.f90:23-300)
 // ---------------------------------------------------------------------------
 
-// Q = 5 for LS-SPH (SPH_MODEL 3), Q = 2 for the classical models.
+// Q = 5 for LS-SPH (SPH_MODEL 3), 9 and 14 for SPH_MODEL 4-5 (FP64), 2 for
+// the classical models.
 template <bool T, bool MIXED, int KT, int Q>
 __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
@@ -454,7 +457,21 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
       hipLaunchKernelGGL((k_lsa<2, false, false, 0, 9>), g, b, lds, s, a);
     return;
   }
-  if (model >= 5) {  // q = 14, 20: one wavefront per particle, FP64
+  if (model == 5) {  // q = 14, FP64, one thread per particle (part of the sums in scratch)
+    const dim3 g(xcd_grid(a.n, d.block)), b(d.block);
+    if (cls == 0)
+      hipLaunchKernelGGL((k_lsa<0, false, false, 0, 14>), g, b, lds, s, a);
+    else if (cls == 1 && delta)
+      hipLaunchKernelGGL((k_lsa<1, false, true, 0, 14>), g, b, lds, s, a);
+    else if (cls == 1)
+      hipLaunchKernelGGL((k_lsa<1, false, false, 0, 14>), g, b, lds, s, a);
+    else if (delta)
+      hipLaunchKernelGGL((k_lsa<2, false, true, 0, 14>), g, b, lds, s, a);
+    else
+      hipLaunchKernelGGL((k_lsa<2, false, false, 0, 14>), g, b, lds, s, a);
+    return;
+  }
+  if (model == 6) {  // q = 20: one wavefront per particle, FP64
     launch_lsa_wave(s, d, a, model, cls);
     return;
   }
@@ -529,7 +546,15 @@ void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
       hipLaunchKernelGGL((k_pst<false, false, 0, 9>), g, b, lds, s, a);
     return;
   }
-  if (model >= 5) {
+  if (model == 5) {  // q = 14, FP64, one thread per particle
+    const dim3 g(xcd_grid(a.n, d.block)), b(d.block);
+    if (d.opt.has_tem)
+      hipLaunchKernelGGL((k_pst<true, false, 0, 14>), g, b, lds, s, a);
+    else
+      hipLaunchKernelGGL((k_pst<false, false, 0, 14>), g, b, lds, s, a);
+    return;
+  }
+  if (model == 6) {  // q = 20: one wavefront per particle
     launch_pst_wave(s, d, a, model);
     return;
   }
