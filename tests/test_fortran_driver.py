@@ -30,6 +30,8 @@ DRIVER_CASES = {
     "tg_rk4_delta_restart": "tg_rk4_delta",
     "diff_rk4_restart": "diff_rk4",
     "bous_rk4_wm3_restart": "bous_rk4_wm3",
+    "tg_sph2_rk4_delta_restart": "tg_sph2_rk4_delta",
+    "tg_ls9_rk2_restart": "tg_ls9_rk2",
 }
 # macros that only set the run's schedule, names and launch mode
 SCHEDULE = {"SAVE_NAME", "READ_NAME", "START_STEP", "END_STEP", "WRITE_STEP", "USE_GRAPH"}
@@ -120,11 +122,14 @@ def test_driver_cavity_new_run(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["tg_rk4_delta_restart", "diff_rk4_restart", "bous_rk4_wm3_restart"])
+@pytest.mark.parametrize("case", ["tg_rk4_delta_restart", "diff_rk4_restart", "bous_rk4_wm3_restart",
+                                  "tg_sph2_rk4_delta_restart", "tg_ls9_rk2_restart"])
 def test_driver_restart(case, tmp_path):
     """Restart from results/inject/data/1.dat (the reference's state0); RK4
     (graph replay for the Taylor-Green and Boussinesq cases, eager calls for
-    the diffusion case, which also runs the steady-state check each step)."""
+    the diffusion case, which also runs the steady-state check each step);
+    also classical SPH (SPH_MODEL 2) and q = 9 (SPH_MODEL 4) from a config.h
+    of the reference."""
     g = Golden(DRIVER_CASES[case])
     write_state(tmp_path / "results" / "inject" / "data" / "1.dat", g.state("state0"), g.has_tem)
     r = run(case, tmp_path)
