@@ -1,17 +1,18 @@
 // operators.hip -- LS-SPH pair operators for gfx950.
 //
-// One thread per mirror slot.  A pair kernel scans the stencil's candidates
-// (x/y only, 16 B each) and compacts the accepted slots into a per-thread
-// list in LDS (lds[k * blockDim + tid], bank-conflict free), then walks that
-// list with every lane busy, loading 64-byte neighbor records; the ~100-flop
-// pair body thus stays out of the 1-in-3 divergent candidate test.  A thread
-// whose list overflows walks the candidates again directly.
+// One thread per slot.  A pair kernel scans the stencil's candidates (x/y
+// only, 16 B each) and compacts the accepted slots into a per-thread list in
+// LDS (lds[k * blockDim + tid], bank-conflict free), then walks that list
+// with every lane busy, loading the 48 B of each neighbor record it uses; the
+// ~100-flop pair body thus stays out of the 1-in-3 divergent candidate test.
+// A thread whose list overflows walks the candidates again directly.
 //
-// The LS type-A kernel (LSSPH_A_mod.f90:26-271) assembles the 5x5 moment
+// The LS type-A kernel (LSSPH_A_mod.f90:26-271) assembles the q x q moment
 // matrix and right-hand sides in registers, solves, forms the RHS of the
 // governing equations and applies the RK2/RK4 stage update (RK2_mod.f90:
 // 83-110, RK4_mod.f90:234-288) in its epilogue, writing the particle's new
-// record to the other physical buffer.
+// record to the same slot of the other record buffer.  The classical-SPH
+// kernel (classical_SPH_mod.f90) shares the scan, the walk and the epilogue.
 //   FP64 path: LU with partial pivoting (DGESV semantics).
 //   Mixed path: FP32 pair sums on FP64-formed offsets and differences, FP64
 //   LDL^T + one refinement step; particles whose LDL^T pivots are not
@@ -303,8 +304,8 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-instance max reductions over the fluid particles of a physical range
-// (an instance occupies the contiguous physical range [pbeg, pend)).
+// Per-instance max reductions over the fluid particles of an instance (its
+// particles occupy the contiguous slot range of its cells).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double block_max(double v) {
   __shared__ double sh[4];
