@@ -195,7 +195,8 @@ int32_t lssph_synchronize(lssph_handle_t h);
 
 /* Tuning knobs (project autotuner): threads per block of the pair kernels,
    on-chip neighbor-list length (entries per particle, the particle itself
-   included; a longer list walks the candidates instead), and how many steps
+   included; a longer list walks the candidates instead; lssph_create sizes
+   it from pi (h_eff/dx)^2), and how many steps
    pass between physical re-sorts of the particle records.  0 leaves a value
    unchanged. */
 int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len,

@@ -478,6 +478,20 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
     for (int k = 0; k < n_inst; ++k) nmax = std::max<int64_t>(nmax, (int64_t)n_int[k] + n_ext[k]);
     d.red_blocks = (int)std::min<int64_t>(256, (nmax + 4095) / 4096);
   }
+  {
+    // on-chip list length from the expected neighbor count pi (h_eff/dx)^2
+    // with a quarter of headroom for disorder (30 -> 39 entries at the
+    // default h_eff = 3.1 dx, 68 -> 86 for the quintic spline's 3 h), within
+    // the 64 KiB of LDS of a block
+    double nmax = 0.0;
+    for (int k = 0; k < n_inst; ++k) {
+      const double r = params[k].h_eff / params[k].dx;
+      nmax = std::max(nmax, kPi * r * r);
+    }
+    const int want = (int)std::ceil(1.25 * nmax) + 1;
+    const int cap = 64 * 1024 / (4 * d.block) - 1;
+    d.list_len = std::min(std::max(d.list_len, want), std::min(cap, 128));
+  }
   // upload index -> instance: fluid blocks, then wall blocks, in instance order
   h->inst_of.resize(d.n_total);
   {
