@@ -1,14 +1,22 @@
 // boundary.hip -- ghost-wall particles for gfx950 (ghost_mod.f90:43-78).
 //
-// One thread per virtual marker (the mirror of a wall particle across the
-// wall): interpolate the fluid values at the marker from its fluid
-// neighbors in the cell-sorted mirror (LS type B with q = 3 or 6, or CSPH),
-// then extrapolate them to the paired wall particle per wall type and
-// boundary condition.  The wall values are written to the wall particle's
+// A group of kGhostLanes lanes per virtual marker (the mirror of a wall
+// particle across the wall): interpolate the fluid values at the marker from
+// its fluid neighbors in the cell-sorted mirror (LS type B with q = 3 or 6,
+// or CSPH), then extrapolate them to the paired wall particle per wall type
+// and boundary condition.  Two lanes per marker halve each lane's serial
+// chain of dependent loads; more lanes lose to the group sums and the
+// repeated solves (100K-marker ensemble: 1, 2, 4, 8 lanes take 69, 49, 51,
+// 58 us per launch).  The wall values are written to the wall particle's
 // slot of the current records (read by the next pair kernel).
 #include "pair_common.hpp"
 
 namespace lssph {
+
+#ifndef LSSPH_GHOST_LANES
+#define LSSPH_GHOST_LANES 2
+#endif
+constexpr int kGhostLanes = LSSPH_GHOST_LANES;
 
 struct GhostArgs {
   Rec *__restrict__ mir;   // current records: fluid read, wall slots written
@@ -32,11 +40,22 @@ __device__ __forceinline__ double wall_pre(const InstDev &I, bool has_t, int iso
   return vpre;
 }
 
+// Sum over the G lanes that share one virtual marker (G a power of two,
+// groups aligned to G lanes).  Every lane of the group ends with the total.
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, G);
+  return v;
+}
+
 // interpolate_LSSPH_B_mod.f90:24-178: a = [1, x/h, y/h (, x^2/2h^2, xy/h^2,
-// y^2/2h^2)], fluid neighbors only, value = constant term of the fit.
-template <int Q>
-__device__ __forceinline__ void interp_lssph_b(const GhostArgs &A, const InstDev &I, int k, int kt, bool has_t,
-                                               double (&val)[4]) {
+// y^2/2h^2)], fluid neighbors only, value = constant term of the fit.  The
+// G lanes of a marker take every G-th candidate of each contiguous cell run
+// and add their partial sums at the end.
+template <int Q, int G>
+__device__ __forceinline__ void interp_lssph_b(const GhostArgs &A, const InstDev &I, int k, int lane, int kt,
+                                               bool has_t, double (&val)[4]) {
   double M[Q * (Q + 1) / 2];
   double b[4][Q];
 #pragma unroll
@@ -49,9 +68,10 @@ __device__ __forceinline__ void interp_lssph_b(const GhostArgs &A, const InstDev
   const double inv_h = I.inv_h, inv_h2 = I.inv_h_sq;
   int num = 0;
   visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int, int) {
-    for (int t = t0; t < t1; ++t) {
-      if (info_ptype(A.sinfo[t]) != 0) continue;
+    for (int t = t0 + lane; t < t1; t += G) {
+      const int32_t inf = A.sinfo[t];
       const double2 xy = load_xy(A.mir, t);
+      if (info_ptype(inf) != 0) continue;
       const double xj = xy.x - mx, yj = xy.y - my;
       const double r2 = sq_dist(xj, yj);
       if (r2 > I.h_eff_sq) continue;
@@ -79,6 +99,13 @@ __device__ __forceinline__ void interp_lssph_b(const GhostArgs &A, const InstDev
       }
     }
   });
+  num = group_sum<G>(num);
+#pragma unroll
+  for (int i = 0; i < Q * (Q + 1) / 2; ++i) M[i] = group_sum<G>(M[i]);
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int i = 0; i < Q; ++i) b[f][i] = group_sum<G>(b[f][i]);
   if (num >= Q) {
     double Am[Q][Q];
 #pragma unroll
@@ -97,16 +124,19 @@ __device__ __forceinline__ void interp_lssph_b(const GhostArgs &A, const InstDev
   }
 }
 
-// interpolate_CSPH_mod.f90:24-130: Shepard average of the fluid neighbors.
-__device__ __forceinline__ void interp_csph(const GhostArgs &A, const InstDev &I, int k, int kt, bool has_t,
-                                            double (&val)[4]) {
+// interpolate_CSPH_mod.f90:24-130: Shepard average of the fluid neighbors,
+// split over G lanes like interp_lssph_b.
+template <int G>
+__device__ __forceinline__ void interp_csph(const GhostArgs &A, const InstDev &I, int k, int lane, int kt,
+                                            bool has_t, double (&val)[4]) {
   const double mx = A.vm_x[k], my = A.vm_y[k];
   double su = 0, sv = 0, sp = 0, st = 0, sw = 0;
   int num = 0;
   visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int, int) {
-    for (int t = t0; t < t1; ++t) {
-      if (info_ptype(A.sinfo[t]) != 0) continue;
+    for (int t = t0 + lane; t < t1; t += G) {
+      const int32_t inf = A.sinfo[t];
       const double2 xy = load_xy(A.mir, t);
+      if (info_ptype(inf) != 0) continue;
       const double xij = mx - xy.x, yij = my - xy.y;
       const double r2 = sq_dist(xij, yij);
       if (r2 > I.h_eff_sq) continue;
@@ -121,6 +151,12 @@ __device__ __forceinline__ void interp_csph(const GhostArgs &A, const InstDev &I
       if (has_t) st = st + vw * nb.tem;
     }
   });
+  num = group_sum<G>(num);
+  sw = group_sum<G>(sw);
+  su = group_sum<G>(su);
+  sv = group_sum<G>(sv);
+  sp = group_sum<G>(sp);
+  st = group_sum<G>(st);
   if (num > 0 && sw > 1.0e-12) {
     val[0] = su / sw;
     val[1] = sv / sw;
@@ -131,19 +167,26 @@ __device__ __forceinline__ void interp_csph(const GhostArgs &A, const InstDev &I
   }
 }
 
+// G lanes per virtual marker; the whole group leaves together when the
+// marker index is past the end, so the group sums read active lanes only.
+// WM is the wall model (1 CSPH, 2 LS q = 3, 3 LS q = 6): one instantiation
+// each, so the q = 6 registers do not lower the occupancy of the others.
+template <int G, int WM>
 __global__ void __launch_bounds__(256) k_ghost(GhostArgs A) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = g / G, lane = g % G;
   if (k >= A.n_ext) return;
   const Options &o = A.opt;
   const InstDev &I = A.inst[A.vm_inst[k]];
   const bool has_t = o.has_tem != 0;
   double val[4];
-  if (o.wall_model == 1)
-    interp_csph(A, I, k, o.kernel_type, has_t, val);
-  else if (o.wall_model == 3)
-    interp_lssph_b<6>(A, I, k, o.kernel_type, has_t, val);
+  if constexpr (WM == 1)
+    interp_csph<G>(A, I, k, lane, o.kernel_type, has_t, val);
+  else if constexpr (WM == 3)
+    interp_lssph_b<6, G>(A, I, k, lane, o.kernel_type, has_t, val);
   else
-    interp_lssph_b<3>(A, I, k, o.kernel_type, has_t, val);
+    interp_lssph_b<3, G>(A, I, k, lane, o.kernel_type, has_t, val);
+  if (lane != 0) return;
   A.vm_u[k] = val[0];
   A.vm_v[k] = val[1];
   A.vm_pre[k] = val[2];
@@ -290,7 +333,14 @@ void launch_ghost(hipStream_t s, Device &d, int cur) {
   a.vm_tem = d.vm_tem;
   a.opt = d.opt;
   a.n_ext = d.n_ext;
-  hipLaunchKernelGGL(k_ghost, dim3(blocks(d.n_ext, 128)), dim3(128), 0, s, a);
+  constexpr int G = kGhostLanes;
+  const dim3 grid(blocks(d.n_ext, 256 / G)), block(256);
+  if (d.opt.wall_model == 1)
+    hipLaunchKernelGGL((k_ghost<G, 1>), grid, block, 0, s, a);
+  else if (d.opt.wall_model == 3)
+    hipLaunchKernelGGL((k_ghost<G, 3>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_ghost<G, 2>), grid, block, 0, s, a);
 }
 
 void launch_vm_init(hipStream_t s, Device &d, int cur) {
