@@ -27,12 +27,6 @@ namespace lssph {
 constexpr int kChunk = 32;   // neighbors staged in LDS per round
 constexpr int kWavesPB = 2;  // wavefronts (particles) per block
 
-// Orders the LDS accesses of the lanes of one wavefront.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Kernel W(r) and dW/dr written term by term as kernel_functions_mod.f90
 // forms them (q = r / h, powers as repeated products, normalisation applied

@@ -266,13 +266,13 @@ __device__ __forceinline__ void scale_derivs(const double (&X)[NF][5], const Ins
 // q = 5, 9 or 14 (SPH_MODEL 3-5).
 template <int CLS, bool DELTA, int KT, int Q = 5>
 __device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
-                                         const int32_t *lds, int cnt, int list_len, int B, int tid,
+                                         const NbList &L,
                                          double (&d)[Layout<CLS, DELTA>::NF][5]) {
   constexpr int NF = Layout<CLS, DELTA>::NF;
   const int kt = A.kt;
   LsSums<double, NF, Q> S;
   S.zero();
-  walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, list_len, B, tid,
+  walk_neighbors(A.mir, A.cstart, I, lc, s, me, L,
                  [&](const Rec &nb, double dx, double dy) {
                    const double r = sqrt(sq_dist(dx, dy));
                    const double w = I.w_norm * kshape_w<double, KT>(kt, r * I.inv_h);
@@ -296,13 +296,13 @@ __device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int
 // Mixed-precision assembly + FP64 LDL^T with refinement; false -> fallback.
 template <int CLS, bool DELTA, int KT>
 __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
-                                         const int32_t *lds, int cnt, int list_len, int B, int tid,
+                                         const NbList &L,
                                          double (&d)[Layout<CLS, DELTA>::NF][5]) {
   constexpr int NF = Layout<CLS, DELTA>::NF;
   const int kt = A.kt;
   LsSums<float, NF> S;
   S.zero();
-  walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, list_len, B, tid,
+  walk_neighbors(A.mir, A.cstart, I, lc, s, me, L,
                  [&](const Rec &nb, double dx, double dy) {
                    const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
                    const float q = fast_sqrtf(xs * xs + ys * ys);

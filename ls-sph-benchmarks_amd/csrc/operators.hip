@@ -2,10 +2,12 @@
 //
 // One thread per slot.  A pair kernel scans the stencil's candidates (x/y
 // only, 16 B each) and compacts the accepted slots into a per-thread list in
-// LDS (lds[k * blockDim + tid], bank-conflict free), then walks that list
-// with every lane busy, loading the 48 B of each neighbor record it uses; the
-// ~100-flop pair body thus stays out of the 1-in-3 divergent candidate test.
-// A thread whose list overflows walks the candidates again directly.
+// LDS (per-wave rows, bank-conflict free), then walks that list with every
+// lane busy, loading the 48 B of each neighbor record it uses; the ~100-flop
+// pair body thus stays out of the 1-in-3 divergent candidate test.  Full
+// wavefronts inside one cell row stage their candidates' positions in LDS
+// with coalesced loads and scan from there.  A thread whose list overflows
+// walks the candidates again directly.
 //
 // The LS type-A kernel (LSSPH_A_mod.f90:26-271) assembles the q x q moment
 // matrix and right-hand sides in registers, solves, forms the RHS of the
@@ -43,7 +45,9 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
   // list entries, the particle itself included
-  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
+  NbList nl = make_list(lds, A.list_len, tid);
+  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl);
+  const int cnt = nl.cnt;
   if (cnt > A.list_len) {
     atomicAdd(&A.stats64[kOverflows], 1ull);
     atomicMax(&A.counters[kMaxNbr], cnt - 1);
@@ -55,13 +59,13 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
     for (int i = 0; i < 5; ++i) d[f][i] = 0.0;
   if (cnt - 1 >= Q) {  // LSSPH_A_mod.f90:208 (num_neighbors >= q)
     if constexpr (MIXED) {
-      if (!derivs32<CLS, DELTA, KT>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d)) {
+      if (!derivs32<CLS, DELTA, KT>(A, I, lc, s, me, nl, d)) {
         const int k = atomicAdd(A.fb_count, 1);
         A.fb_list[k] = s;
         return;
       }
     } else {
-      derivs64<CLS, DELTA, KT, Q>(A, I, lc, s, me, lds, cnt, A.list_len, B, tid, d);
+      derivs64<CLS, DELTA, KT, Q>(A, I, lc, s, me, nl, d);
     }
   }
   // Re-read the particle's own record for the epilogue instead of keeping it
@@ -83,7 +87,7 @@ __global__ void __launch_bounds__(256) k_lsa_fallback(LsaArgs A) {
     const Rec me = A.mir[s];
     const int lc = local_cell(I, me.x, me.y);
     double d[NF][5];
-    derivs64<CLS, DELTA, KT>(A, I, lc, s, me, nullptr, 1 << 30, 0, 0, 0, d);
+    derivs64<CLS, DELTA, KT>(A, I, lc, s, me, no_list(), d);
     atomicAdd(&A.stats64[kFallbacks], 1ull);
     finish<CLS, DELTA>(A, I, A.items[s], s, me, d);
   }
@@ -114,7 +118,9 @@ __global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
   const InstDev &I = A.inst[info_inst(inf)];
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
-  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
+  NbList nl = make_list(lds, A.list_len, tid);
+  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl);
+  const int cnt = nl.cnt;
   if (cnt > A.list_len) {
     atomicAdd(&A.stats64[kOverflows], 1ull);
     atomicMax(&A.counters[kMaxNbr], cnt - 1);
@@ -122,7 +128,7 @@ __global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
   const int kt = A.kt;
   double du_dx = 0.0, dv_dy = 0.0, dp_dx = 0.0, dp_dy = 0.0;
   double lap_u = 0.0, lap_v = 0.0, lap_t = 0.0, lap_rho = 0.0;
-  walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, A.list_len, B, tid,
+  walk_neighbors(A.mir, A.cstart, I, lc, s, me, nl,
                  [&](const Rec &nb, double dx, double dy) {
                    const double r = sqrt(sq_dist(dx, dy));
                    if (!(r > 0.0)) return;  // the particle itself
@@ -190,14 +196,16 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   const int lc = local_cell(I, me.x, me.y);
   const int kt = A.kt;
   // list entries, the particle itself included
-  const int cnt = scan_candidates(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, lds, A.list_len, B, tid);
+  NbList nl = make_list(lds, A.list_len, tid);
+  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl);
+  const int cnt = nl.cnt;
   if (cnt > A.list_len) atomicAdd(&A.stats64[kOverflows], 1ull);
 
   LsSums<R, NF, Q> S;
   S.zero();
   R sx = R(0), sy = R(0);
   const double inv_w_ave = 1.0 / I.w_ave;
-  walk_neighbors(A.mir, A.cstart, I, lc, s, me, lds, cnt, A.list_len, B, tid,
+  walk_neighbors(A.mir, A.cstart, I, lc, s, me, nl,
                  [&](const Rec &nb, double dx, double dy) {
                    R df[NF];
                    df[0] = R(nb.u - me.u);

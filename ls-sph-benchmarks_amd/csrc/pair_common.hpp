@@ -18,6 +18,51 @@ namespace lssph {
 constexpr int kSlotBits = 27;
 constexpr int kSlotMask = (1 << kSlotBits) - 1;
 
+// A lane's neighbor list in the block's LDS.  Each wavefront owns
+// list_len + 1 rows of 256 bytes (4 * (list_len + 1) * blockDim bytes per
+// block in all), and lane l's entry k sits at byte 256 k + 4 l of its
+// wave's rows (consecutive lanes, consecutive banks).  Row list_len is the
+// scan's trash entry.  A wave whose scan was staged (scan_staged) stores
+// 16-bit entries instead, slot - base, at byte 128 k + 2 l, and uses the
+// upper half of its rows to stage candidate positions.
+struct NbList {
+  char *wbase;   // the wave's rows
+  int lane;      // lane within the wavefront
+  int list_len;  // entries before the trash entry
+  int cnt;       // entries, the particle itself included; > list_len: overflowed
+  int base;      // >= 0: 16-bit entries relative to base; -1: 32-bit entries
+};
+
+__device__ __forceinline__ NbList make_list(int32_t *lds, int list_len, int tid) {
+  NbList L;
+  L.wbase = reinterpret_cast<char *>(lds) + (size_t)(tid >> 6) * 256u * (unsigned)(list_len + 1);
+  L.lane = tid & 63;
+  L.list_len = list_len;
+  L.cnt = 0;
+  L.base = -1;
+  return L;
+}
+
+// A list that is never read: every walk takes the direct stencil walk.
+__device__ __forceinline__ NbList no_list() {
+  NbList L;
+  L.wbase = nullptr;
+  L.lane = 0;
+  L.list_len = 0;
+  L.cnt = 1 << 30;
+  L.base = -1;
+  return L;
+}
+
+// Orders the LDS accesses of the lanes of one wavefront (LDS operations of
+// a wavefront complete in issue order; the fences keep the compiler from
+// moving accesses across this point).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Image shift of a wrapped periodic cell: code 1 -> -len, 2 -> +len.
 __device__ __forceinline__ double shift_of(int code, double len) {
   return code == 1 ? -len : (code == 2 ? len : 0.0);
@@ -139,13 +184,13 @@ __device__ __forceinline__ int image_code(double sx, double sy) {
 // first test.
 __device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
                                                const InstDev &I, int lc, double xi, double yi, double r2,
-                                               int32_t *lds, int list_len, int B, int tid) {
+                                               const NbList &L) {
   int cnt = 0;
-  char *lbase = reinterpret_cast<char *>(lds) + 4 * tid;
-  const unsigned B4 = 4u * (unsigned)B;
+  char *lbase = L.wbase + 4 * L.lane;
+  const unsigned list_len = (unsigned)L.list_len;
   auto accept = [&](int t, double dx, double dy, int code) {
-    const unsigned k = min((unsigned)cnt, (unsigned)list_len);
-    *reinterpret_cast<int32_t *>(lbase + __umul24(k, B4)) = t | (code << kSlotBits);
+    const unsigned k = min((unsigned)cnt, list_len);
+    *reinterpret_cast<int32_t *>(lbase + (k << 8)) = t | (code << kSlotBits);
     cnt += (sq_dist(dx, dy) <= r2) ? 1 : 0;
   };
   if (!wave_wraps(I, lc)) {
@@ -182,6 +227,85 @@ __device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, cons
   return cnt;
 }
 
+// Wave-staged candidate scan.  The lanes of a full wavefront whose
+// stencils share one instance and one cell row and do not wrap read the
+// union of their candidate rows (three contiguous slot ranges) once,
+// coalesced, into LDS, and each lane then tests its own window from there:
+// one 16-byte load per lane per 64 candidates instead of one per lane per
+// candidate.  The accepted entries and their order are those of
+// scan_candidates.  Returns false, with the list untouched, when the wave
+// does not qualify (partial or mixed waves, periodic edges, a row union
+// larger than the staging area, a slot span beyond 16 bits).
+__device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
+                                            const InstDev &I, int lc, double xi, double yi, double r2,
+                                            NbList &L) {
+  if (__builtin_amdgcn_read_exec() != ~0ull) return false;
+  const int nx = __builtin_amdgcn_readfirstlane(I.nx);
+  const int ny = __builtin_amdgcn_readfirstlane(I.ny);
+  const int off = __builtin_amdgcn_readfirstlane(I.cell_off);
+  const int cx = lc % I.nx, cy = lc / I.nx;
+  const int cy0 = __builtin_amdgcn_readfirstlane(cy);
+  if (__ballot(I.cell_off != off || cy != cy0 || stencil_wraps(I, lc)) != 0ull) return false;
+  int mn = cx, mx = cx;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    mn = min(mn, __shfl_xor(mn, m));
+    mx = max(mx, __shfl_xor(mx, m));
+  }
+  const int x0 = max(__builtin_amdgcn_readfirstlane(mn) - 1, 0);
+  const int x1 = min(__builtin_amdgcn_readfirstlane(mx) + 1, nx - 1);
+  const int y0 = max(cy0 - 1, 0), y1 = min(cy0 + 1, ny - 1);
+  const int cap = 8 * (L.list_len + 1);  // 16-byte positions in the upper half of the wave's rows
+  const int base = cstart[off + y0 * nx + x0];
+  if (cstart[off + y1 * nx + x1 + 1] - base > 0xffff) return false;
+  for (int yy = y0; yy <= y1; ++yy)
+    if (cstart[off + yy * nx + x1 + 1] - cstart[off + yy * nx + x0] > cap) return false;
+
+  const int lane = L.lane;
+  char *l16 = L.wbase + 2 * lane;
+  double2 *stage = reinterpret_cast<double2 *>(L.wbase + 128 * (L.list_len + 1));
+  const unsigned list_len = (unsigned)L.list_len;
+  const int lx0 = max(cx - 1, 0), lx1 = min(cx + 1, nx - 1);
+  int cnt = 0;
+  auto accept = [&](int t, double2 p) {
+    const unsigned k = min((unsigned)cnt, list_len);
+    *reinterpret_cast<uint16_t *>(l16 + (k << 7)) = (uint16_t)(t - base);
+    cnt += (sq_dist(p.x - xi, p.y - yi) <= r2) ? 1 : 0;
+  };
+#pragma unroll 1
+  for (int yy = y0; yy <= y1; ++yy) {
+    const int row = off + yy * nx;
+    const int u0 = cstart[row + x0], u1 = cstart[row + x1 + 1];
+    const int a0 = cstart[row + lx0], a1 = cstart[row + lx1 + 1];
+    wave_sync();  // the previous row's reads of the staging area come first
+    for (int i = lane; i < u1 - u0; i += 64) stage[i] = load_xy(mir, u0 + i);
+    wave_sync();
+    int t = a0;
+    for (; t + 4 <= a1; t += 4) {
+      const double2 *sp = stage + (t - u0);
+      const double2 p0 = sp[0], p1 = sp[1], p2 = sp[2], p3 = sp[3];
+      accept(t, p0);
+      accept(t + 1, p1);
+      accept(t + 2, p2);
+      accept(t + 3, p3);
+    }
+    for (; t < a1; ++t) accept(t, stage[t - u0]);
+  }
+  L.cnt = cnt;
+  L.base = base;
+  return true;
+}
+
+// Fills the lane's list: the staged scan when the wave qualifies, else the
+// per-lane scan.
+__device__ __forceinline__ void build_list(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
+                                           const InstDev &I, int lc, double xi, double yi, double r2, NbList &L) {
+  if (!scan_staged(mir, cstart, I, lc, xi, yi, r2, L)) {
+    L.base = -1;
+    L.cnt = scan_candidates(mir, cstart, I, lc, xi, yi, r2, L);
+  }
+}
+
 // Calls f(nb_record, dx, dy) for every neighbor within h_eff of slot s, in
 // stencil order, by a walk over the candidates of the cell list.
 template <typename F>
@@ -206,21 +330,29 @@ __device__ __forceinline__ void walk_stencil(const Rec *__restrict__ mir, const 
 // by a walk over the candidates, which skips it.
 template <typename F>
 __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
-                                               const InstDev &I, int lc, int s, const Rec &me,
-                                               const int32_t *lds, int cnt, int list_len, int B, int tid,
+                                               const InstDev &I, int lc, int s, const Rec &me, const NbList &L,
                                                F &&f) {
-  if (cnt > list_len) {
+  const int cnt = L.cnt;
+  if (cnt > L.list_len) {
     walk_stencil(mir, cstart, I, lc, s, me, f);
     return;
   }
-  if (!wave_wraps(I, lc)) {
+  if (L.base >= 0) {
+    const char *l16 = L.wbase + 2 * L.lane;
     for (int k = 0; k < cnt; ++k) {
-      const Rec nb = mir[lds[k * B + tid] & kSlotMask];
+      const Rec nb = mir[L.base + *reinterpret_cast<const uint16_t *>(l16 + (k << 7))];
+      f(nb, nb.x - me.x, nb.y - me.y);
+    }
+  } else if (!wave_wraps(I, lc)) {
+    const char *l32 = L.wbase + 4 * L.lane;
+    for (int k = 0; k < cnt; ++k) {
+      const Rec nb = mir[*reinterpret_cast<const int32_t *>(l32 + (k << 8)) & kSlotMask];
       f(nb, nb.x - me.x, nb.y - me.y);
     }
   } else {
+    const char *l32 = L.wbase + 4 * L.lane;
     for (int k = 0; k < cnt; ++k) {
-      const int e = lds[k * B + tid];
+      const int e = *reinterpret_cast<const int32_t *>(l32 + (k << 8));
       const int t = e & kSlotMask, code = e >> kSlotBits;
       const Rec nb = mir[t];
       const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
