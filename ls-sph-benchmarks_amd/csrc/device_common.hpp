@@ -232,6 +232,15 @@ __device__ __forceinline__ int lu_solve(T (&a)[N][N], T (&b)[NR][N]) {
   return info;
 }
 
+// Instance constants through the constant address space: the memory is not
+// written while kernels run, so loads at a wave-uniform address (instance
+// index from readfirstlane) become scalar loads instead of per-lane vector
+// loads.
+__device__ __forceinline__ const InstDev *inst_const(const InstDev *p, int i) {
+  using CInst = const __attribute__((address_space(4))) InstDev;
+  return (const InstDev *)((CInst *)p + i);
+}
+
 // XCD-aware block index: the hardware deals workgroups round-robin to the 8
 // XCDs (block b runs on XCD b % 8), each with its own L2.  Remapping gives
 // XCD k the contiguous logical blocks [k * G/8, (k+1) * G/8), so the slot

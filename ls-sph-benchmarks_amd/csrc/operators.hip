@@ -27,21 +27,9 @@ namespace lssph {
 // Q = 5 (SPH_MODEL 3), 9 or 14 (SPH_MODEL 4-5, FP64; at q = 14 part of the
 // sums live in scratch, still several times faster than one wavefront per
 // particle).
-template <int CLS, bool MIXED, bool DELTA, int KT, int Q = 5>
-__global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
-  static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
+template <int CLS, bool MIXED, bool DELTA, int KT, int Q>
+__device__ __forceinline__ void lsa_particle(const LsaArgs &A, const InstDev &I, int s, int tid, int32_t *lds) {
   constexpr int NF = Layout<CLS, DELTA>::NF;
-  extern __shared__ int32_t lds[];
-  const int tid = threadIdx.x, B = blockDim.x;
-  const int s = xcd_block() * B + tid;
-  if (s >= A.n) return;
-  const int32_t inf = A.sinfo[s];
-  if (info_ptype(inf) != 0) {
-    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];  // walls move on with the records
-    if (A.spd) A.spd[s] = 0.0;
-    return;
-  }
-  const InstDev &I = A.inst[info_inst(inf)];
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
   // list entries, the particle itself included
@@ -76,6 +64,28 @@ __global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
   finish<CLS, DELTA>(A, I, A.items[s], s, me2, d);
 }
 
+template <int CLS, bool MIXED, bool DELTA, int KT, int Q = 5>
+__global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
+  static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
+  extern __shared__ int32_t lds[];
+  const int tid = threadIdx.x, B = blockDim.x;
+  const int s = xcd_block() * B + tid;
+  if (s >= A.n) return;
+  const int32_t inf = A.sinfo[s];
+  if (info_ptype(inf) != 0) {
+    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];  // walls move on with the records
+    if (A.spd) A.spd[s] = 0.0;
+    return;
+  }
+  // A wave of one instance (the common case) reads the instance constants
+  // with scalar loads; a wave that spans instances, with per-lane loads.
+  const int ii = info_inst(inf), ii0 = __builtin_amdgcn_readfirstlane(ii);
+  if (__ballot(ii != ii0) == 0ull)
+    lsa_particle<CLS, MIXED, DELTA, KT, Q>(A, *inst_const(A.inst, ii0), s, tid, lds);
+  else
+    lsa_particle<CLS, MIXED, DELTA, KT, Q>(A, A.inst[ii], s, tid, lds);
+}
+
 // FP64 redo of the slots the mixed kernel queued (direct neighbor walk).
 template <int CLS, bool DELTA, int KT>
 __global__ void __launch_bounds__(256) k_lsa_fallback(LsaArgs A) {
@@ -102,20 +112,9 @@ __global__ void __launch_bounds__(256) k_lsa_fallback(LsaArgs A) {
 // d[U][0] = du/dx, d[V][1] = dv/dy, d[P][0..1] = grad p, d[*][2] = Laplacian.
 // ---------------------------------------------------------------------------
 template <int CLS, bool DELTA, bool SUM>
-__global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
+__device__ __forceinline__ void csph_particle(const LsaArgs &A, const InstDev &I, int s, int tid, int32_t *lds) {
   using L = Layout<CLS, DELTA>;
   constexpr int NF = L::NF;
-  extern __shared__ int32_t lds[];
-  const int tid = threadIdx.x, B = blockDim.x;
-  const int s = xcd_block() * B + tid;
-  if (s >= A.n) return;
-  const int32_t inf = A.sinfo[s];
-  if (info_ptype(inf) != 0) {
-    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];
-    if (A.spd) A.spd[s] = 0.0;
-    return;
-  }
-  const InstDev &I = A.inst[info_inst(inf)];
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
   NbList nl = make_list(lds, A.list_len, tid);
@@ -170,6 +169,26 @@ __global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
   finish<CLS, DELTA>(A, I, A.items[s], s, me2, d);
 }
 
+
+template <int CLS, bool DELTA, bool SUM>
+__global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
+  extern __shared__ int32_t lds[];
+  const int tid = threadIdx.x, B = blockDim.x;
+  const int s = xcd_block() * B + tid;
+  if (s >= A.n) return;
+  const int32_t inf = A.sinfo[s];
+  if (info_ptype(inf) != 0) {
+    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];
+    if (A.spd) A.spd[s] = 0.0;
+    return;
+  }
+  const int ii = info_inst(inf), ii0 = __builtin_amdgcn_readfirstlane(ii);
+  if (__ballot(ii != ii0) == 0ull)
+    csph_particle<CLS, DELTA, SUM>(A, *inst_const(A.inst, ii0), s, tid, lds);
+  else
+    csph_particle<CLS, DELTA, SUM>(A, A.inst[ii], s, tid, lds);
+}
+
 // ---------------------------------------------------------------------------
 // Particle shifting (PST_mod.f90:37-114, shift_interpolate_mod.f90:23-300)
 // ---------------------------------------------------------------------------
@@ -177,21 +196,10 @@ __global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
 // Q = 5 for LS-SPH (SPH_MODEL 3), 9 and 14 for SPH_MODEL 4-5 (FP64), 2 for
 // the classical models.
 template <bool T, bool MIXED, int KT, int Q>
-__global__ void __launch_bounds__(256) k_pst(PstArgs A) {
-  static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
+__device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I, int ii, int s, int tid,
+                                             int32_t *lds) {
   constexpr int NF = T ? 4 : 3;  // u, v, rho (, tem)
   using R = typename std::conditional<MIXED, float, double>::type;
-  extern __shared__ int32_t lds[];
-  const int tid = threadIdx.x, B = blockDim.x;
-  const int s = xcd_block() * B + tid;
-  if (s >= A.n) return;
-  const int32_t inf = A.sinfo[s];
-  if (info_ptype(inf) != 0) {
-    A.mout[s] = A.mir[s];  // walls move on with the records
-    return;
-  }
-  const int ii = info_inst(inf);
-  const InstDev &I = A.inst[ii];
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
   const int kt = A.kt;
@@ -309,6 +317,28 @@ __global__ void __launch_bounds__(256) k_pst(PstArgs A) {
   out.vol = I.sp_mass / out.rho;
   A.mout[s] = out;
   record_move(A.mv, I, s, out.x, out.y);
+}
+
+
+// The mixed kernel is held to 128 VGPRs (4 waves per SIMD); its two
+// instance paths otherwise need 129.
+template <bool T, bool MIXED, int KT, int Q>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 4 : 1))) k_pst(PstArgs A) {
+  static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
+  extern __shared__ int32_t lds[];
+  const int tid = threadIdx.x, B = blockDim.x;
+  const int s = xcd_block() * B + tid;
+  if (s >= A.n) return;
+  const int32_t inf = A.sinfo[s];
+  if (info_ptype(inf) != 0) {
+    A.mout[s] = A.mir[s];  // walls move on with the records
+    return;
+  }
+  const int ii = info_inst(inf), ii0 = __builtin_amdgcn_readfirstlane(ii);
+  if (__ballot(ii != ii0) == 0ull)
+    pst_particle<T, MIXED, KT, Q>(A, *inst_const(A.inst, ii0), ii0, s, tid, lds);
+  else
+    pst_particle<T, MIXED, KT, Q>(A, A.inst[ii], ii, s, tid, lds);
 }
 
 // ---------------------------------------------------------------------------
