@@ -214,6 +214,14 @@ int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len,
    does.  Call before lssph_upload. */
 int32_t lssph_set_skin(lssph_handle_t h, double skin);
 
+/* Candidate scan of the pair kernels (no reference counterpart).  1 (the
+   default): a full wavefront whose particles lie in one cell row loads its
+   candidates' positions once into on-chip memory and scans them there;
+   other wavefronts scan per particle.  0: every wavefront scans per
+   particle.  Both give the same neighbor lists in the same order, so the
+   results are bitwise identical. */
+int32_t lssph_set_scan(lssph_handle_t h, int32_t staged);
+
 /* The HIP stream (hipStream_t) the handle launches on, for event timing. */
 void *lssph_stream(lssph_handle_t h);
 

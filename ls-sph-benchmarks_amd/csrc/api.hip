@@ -898,6 +898,14 @@ int32_t lssph_set_skin(lssph_handle_t h, double skin) {
   return 0;
 }
 
+int32_t lssph_set_scan(lssph_handle_t h, int32_t staged) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (staged != 0 && staged != 1) return fail(h, LSSPH_E_ARG, "lssph_set_scan: staged must be 0 or 1");
+  h->d.staged_scan = staged;
+  drop_graphs(h);
+  return 0;
+}
+
 void *lssph_stream(lssph_handle_t h) { return h ? (void *)h->stream : nullptr; }
 
 int32_t lssph_profile(lssph_handle_t h, int32_t enable) {

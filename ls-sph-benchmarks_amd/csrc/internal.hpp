@@ -81,6 +81,7 @@ struct Device {
   int32_t *stage_i32 = nullptr;  // [n_total]
   // tuning
   int block = 128;
+  int staged_scan = 1;  // lssph_set_scan
   int list_len = 39;     // LDS list entries per thread, the particle itself included
   int red_blocks = 256;  // blocks per instance of the max reductions
 };

@@ -247,6 +247,7 @@ struct LsaArgs {
   int32_t *fb_count;
   MoveOut mv;
   int n, kt, list_len, mode;
+  int staged;  // wave-staged candidate scan allowed
 };
 
 template <int NF>
@@ -470,6 +471,7 @@ struct PstArgs {
   unsigned long long *stats64;
   MoveOut mv;
   int n, kt, list_len;
+  int staged;  // wave-staged candidate scan allowed
 };
 
 // The LS operators of SPH_MODEL 6 (q = 20), one wavefront per particle

@@ -34,7 +34,7 @@ __device__ __forceinline__ void lsa_particle(const LsaArgs &A, const InstDev &I,
   const int lc = local_cell(I, me.x, me.y);
   // list entries, the particle itself included
   NbList nl = make_list(lds, A.list_len, tid);
-  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl);
+  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl, A.staged != 0);
   const int cnt = nl.cnt;
   if (cnt > A.list_len) {
     atomicAdd(&A.stats64[kOverflows], 1ull);
@@ -118,7 +118,7 @@ __device__ __forceinline__ void csph_particle(const LsaArgs &A, const InstDev &I
   const Rec me = A.mir[s];
   const int lc = local_cell(I, me.x, me.y);
   NbList nl = make_list(lds, A.list_len, tid);
-  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl);
+  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl, A.staged != 0);
   const int cnt = nl.cnt;
   if (cnt > A.list_len) {
     atomicAdd(&A.stats64[kOverflows], 1ull);
@@ -205,7 +205,7 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
   const int kt = A.kt;
   // list entries, the particle itself included
   NbList nl = make_list(lds, A.list_len, tid);
-  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl);
+  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl, A.staged != 0);
   const int cnt = nl.cnt;
   if (cnt > A.list_len) atomicAdd(&A.stats64[kOverflows], 1ull);
 
@@ -466,6 +466,7 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
   a.n = d.n_total;
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;
+  a.staged = d.staged_scan;
   a.mode = mode;
   const size_t lds = list_lds_bytes(d.list_len, d.block);
   const int cls = d.opt.target_problem == 1 ? 0 : (d.opt.has_tem ? 2 : 1);
@@ -566,6 +567,7 @@ void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
   a.n = d.n_total;
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;
+  a.staged = d.staged_scan;
   const size_t lds = list_lds_bytes(d.list_len, d.block);
   const int kt = d.opt.kernel_type;
   const int model = d.opt.sph_model;

@@ -296,11 +296,12 @@ __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const i
   return true;
 }
 
-// Fills the lane's list: the staged scan when the wave qualifies, else the
-// per-lane scan.
+// Fills the lane's list: the staged scan when it is allowed and the wave
+// qualifies, else the per-lane scan.
 __device__ __forceinline__ void build_list(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
-                                           const InstDev &I, int lc, double xi, double yi, double r2, NbList &L) {
-  if (!scan_staged(mir, cstart, I, lc, xi, yi, r2, L)) {
+                                           const InstDev &I, int lc, double xi, double yi, double r2, NbList &L,
+                                           bool staged) {
+  if (!staged || !scan_staged(mir, cstart, I, lc, xi, yi, r2, L)) {
     L.base = -1;
     L.cnt = scan_candidates(mir, cstart, I, lc, xi, yi, r2, L);
   }

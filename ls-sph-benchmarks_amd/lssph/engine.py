@@ -51,7 +51,7 @@ EXPORTS = ("lssph_create", "lssph_destroy", "lssph_last_error", "lssph_upload", 
            "lssph_download_rhs", "lssph_rk2", "lssph_rk4", "lssph_pst", "lssph_step",
            "lssph_check_steady_state", "lssph_download_vm", "lssph_download_cell_index",
            "lssph_neighbors", "lssph_stats", "lssph_reset_stats", "lssph_synchronize",
-           "lssph_set_tuning", "lssph_set_skin", "lssph_stream", "lssph_profile", "lssph_kernel_times")
+           "lssph_set_tuning", "lssph_set_skin", "lssph_set_scan", "lssph_stream", "lssph_profile", "lssph_kernel_times")
 
 
 def load_library(path: Path | None = None):
@@ -84,6 +84,7 @@ def load_library(path: Path | None = None):
     lib.lssph_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     lib.lssph_set_tuning.argtypes = [vp, i32, i32, i32]
     lib.lssph_set_skin.argtypes = [vp, ctypes.c_double]
+    lib.lssph_set_scan.argtypes = [vp, i32]
     lib.lssph_stream.restype = vp
     lib.lssph_stream.argtypes = [vp]
     lib.lssph_profile.argtypes = [vp, i32]
@@ -270,6 +271,10 @@ class Simulation:
     def set_skin(self, skin: float):
         """Cell-list skin as a fraction of h_eff (0: rebuild after every pass)."""
         self._check(self.lib.lssph_set_skin(self.h, ctypes.c_double(skin)))
+
+    def set_scan(self, staged: bool):
+        """Wave-staged candidate scan on (default) or off; results are bitwise identical."""
+        self._check(self.lib.lssph_set_scan(self.h, 1 if staged else 0))
 
     def profile(self, enable: bool = True):
         self._check(self.lib.lssph_profile(self.h, 1 if enable else 0))

@@ -82,3 +82,8 @@ def test_mismatched_ensemble_options_rejected():
 def test_params_struct_layout_matches_header():
     # 29 doubles then 18 int32: 232 + 72 = 304 bytes, no padding
     assert ctypes.sizeof(lcfg.Params) == 29 * 8 + 18 * 4
+
+
+def test_set_scan_rejects_a_null_handle():
+    lib = engine.load_library()
+    assert lib.lssph_set_scan(None, 1) == -1
