@@ -196,9 +196,9 @@ int32_t lssph_synchronize(lssph_handle_t h);
 /* Tuning knobs (project autotuner): threads per block of the pair kernels,
    on-chip neighbor-list length (entries per particle, the particle itself
    included; a longer list walks the candidates instead; lssph_create sizes
-   it from pi (h_eff/dx)^2), and how many steps
-   pass between physical re-sorts of the particle records.  0 leaves a value
-   unchanged. */
+   it from pi (h_eff/dx)^2), and how many time steps (lssph_step, or
+   lssph_rk2/lssph_rk4 calls) pass between physical re-sorts of the particle
+   records (default 50; negative: never).  0 leaves a value unchanged. */
 int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len,
                          int32_t reorder_interval);
 

@@ -701,10 +701,21 @@ int32_t lssph_download_rhs(lssph_handle_t h, double *rhs_u, double *rhs_v, doubl
   return 0;
 }
 
+// The periodic physical re-sort at the start of a time step (no RK
+// accumulator holds state there); both the eager RK calls and lssph_step
+// count their steps toward it.
+static int maybe_reorder(lssph_handle *h) {
+  if (h->reorder_interval > 0 && !is_diffusion(h) && h->steps_since_reorder >= h->reorder_interval)
+    return reorder(h);
+  return 0;
+}
+
 int32_t lssph_rk2(lssph_handle_t h) {
   if (int rc = ready(h)) return rc;
   if (h->d.opt.rk != 2) return fail(h, LSSPH_E_ARG, "handle was created with rk = 4");
+  if (int rc = maybe_reorder(h)) return rc;
   enqueue_rk(h);
+  ++h->steps_since_reorder;
   HIP_TRY(h, hipGetLastError());
   return 0;
 }
@@ -712,7 +723,9 @@ int32_t lssph_rk2(lssph_handle_t h) {
 int32_t lssph_rk4(lssph_handle_t h) {
   if (int rc = ready(h)) return rc;
   if (h->d.opt.rk != 4) return fail(h, LSSPH_E_ARG, "handle was created with rk = 2");
+  if (int rc = maybe_reorder(h)) return rc;
   enqueue_rk(h);
+  ++h->steps_since_reorder;
   HIP_TRY(h, hipGetLastError());
   return 0;
 }
@@ -728,9 +741,7 @@ int32_t lssph_step(lssph_handle_t h, int32_t nsteps) {
   if (int rc = ready(h)) return rc;
   if (nsteps < 0) return fail(h, LSSPH_E_ARG, "nsteps must be >= 0");
   for (int32_t i = 0; i < nsteps; ++i) {
-    if (h->reorder_interval > 0 && !is_diffusion(h) && h->steps_since_reorder >= h->reorder_interval) {
-      if (int rc = reorder(h)) return rc;
-    }
+    if (int rc = maybe_reorder(h)) return rc;
     // the captured step depends on the records buffer, the build buffers and
     // whether the cell list is fresh
     if (h->passes != 0) full_build(h, 0);

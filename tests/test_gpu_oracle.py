@@ -65,6 +65,20 @@ def test_reorder_does_not_change_results():
             assert np.array_equal(da[k], db[k]), k
 
 
+def test_eager_steps_reorder_without_changing_results():
+    """Eager RK + PST calls also re-sort the records every reorder_interval
+    steps; the results are bitwise those without re-sorts."""
+    param, st = workloads.taylor_green_periodic(n=32, re=100.0, rk=4, jitter=0.2, seed=11)
+    with simulation_from_state(param, st) as a, simulation_from_state(param, st) as b:
+        a.set_tuning(reorder_interval=2)
+        b.set_tuning(reorder_interval=1000)
+        a.eager_step(7)
+        b.eager_step(7)
+        da, db = a.download(), b.download()
+        for k in ("x", "y", "u", "v", "rho", "pre"):
+            assert np.array_equal(da[k], db[k]), k
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_staged_scan_matches_per_lane_scan_bitwise(precision):
     """The wave-staged and the per-lane candidate scans build the same lists
