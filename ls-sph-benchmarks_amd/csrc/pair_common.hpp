@@ -228,11 +228,11 @@ __device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, cons
 }
 
 // Wave-staged candidate scan.  The lanes of a full wavefront whose
-// stencils share one instance and one cell row and do not wrap read the
-// union of their candidate rows (three contiguous slot ranges) once,
-// coalesced, into LDS, and each lane then tests its own window from there:
-// one 16-byte load per lane per 64 candidates instead of one per lane per
-// candidate.  The accepted entries and their order are those of
+// stencils share one instance and at most two adjacent cell rows and do not
+// wrap read the union of their candidate rows (three or four contiguous
+// slot ranges) once, coalesced, into LDS, and each lane then tests its own
+// window from there: one 16-byte load per lane per 64 candidates instead of
+// one per lane per candidate.  The accepted entries and their order are those of
 // scan_candidates.  Returns false, with the list untouched, when the wave
 // does not qualify (partial or mixed waves, periodic edges, a row union
 // larger than the staging area, a slot span beyond 16 bits).
@@ -244,17 +244,22 @@ __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const i
   const int ny = __builtin_amdgcn_readfirstlane(I.ny);
   const int off = __builtin_amdgcn_readfirstlane(I.cell_off);
   const int cx = lc % I.nx, cy = lc / I.nx;
-  const int cy0 = __builtin_amdgcn_readfirstlane(cy);
-  if (__ballot(I.cell_off != off || cy != cy0 || stencil_wraps(I, lc)) != 0ull) return false;
-  int mn = cx, mx = cx;
+  if (__ballot(I.cell_off != off || stencil_wraps(I, lc)) != 0ull) return false;
+  int mn = cx, mx = cx, ymn = cy, ymx = cy;
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
     mn = min(mn, __shfl_xor(mn, m));
     mx = max(mx, __shfl_xor(mx, m));
+    ymn = min(ymn, __shfl_xor(ymn, m));
+    ymx = max(ymx, __shfl_xor(ymx, m));
   }
+  // particles that crossed into the next cell row since the last build put
+  // a wave on two rows: four candidate rows, three of them per lane
+  const int cy0 = __builtin_amdgcn_readfirstlane(ymn), cy1 = __builtin_amdgcn_readfirstlane(ymx);
+  if (cy1 - cy0 > 1) return false;
   const int x0 = max(__builtin_amdgcn_readfirstlane(mn) - 1, 0);
   const int x1 = min(__builtin_amdgcn_readfirstlane(mx) + 1, nx - 1);
-  const int y0 = max(cy0 - 1, 0), y1 = min(cy0 + 1, ny - 1);
+  const int y0 = max(cy0 - 1, 0), y1 = min(cy1 + 1, ny - 1);
   const int cap = 8 * (L.list_len + 1);  // 16-byte positions in the upper half of the wave's rows
   const int base = cstart[off + y0 * nx + x0];
   if (cstart[off + y1 * nx + x1 + 1] - base > 0xffff) return false;
@@ -276,7 +281,8 @@ __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const i
   for (int yy = y0; yy <= y1; ++yy) {
     const int row = off + yy * nx;
     const int u0 = cstart[row + x0], u1 = cstart[row + x1 + 1];
-    const int a0 = cstart[row + lx0], a1 = cstart[row + lx1 + 1];
+    const int a0 = cstart[row + lx0];
+    const int a1 = (yy >= cy - 1 && yy <= cy + 1) ? cstart[row + lx1 + 1] : a0;
     wave_sync();  // the previous row's reads of the staging area come first
     for (int i = lane; i < u1 - u0; i += 64) stage[i] = load_xy(mir, u0 + i);
     wave_sync();
