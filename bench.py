@@ -171,11 +171,11 @@ def copy_probe(nbytes: int) -> None:
     print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}), flush=True)
 
 
-def field_diff(a: dict, b: dict, n: int) -> dict:
+def field_diff(a: dict, b: dict, n: int, fields=("x", "y", "u", "v", "rho", "pre"), periodic: bool = True) -> dict:
     out = {}
-    for k in ("x", "y", "u", "v", "rho", "pre"):
+    for k in fields:
         d = a[k][:n] - b[k][:n]
-        if k in ("x", "y"):
+        if periodic and k in ("x", "y"):
             d = (d + 0.5) % 1.0 - 0.5  # periodic positions
         out[k] = {"max": float(np.max(np.abs(d))), "l2": float(np.sqrt(np.mean(d * d)))}
     return out
@@ -209,6 +209,118 @@ def model_bench(args) -> None:
         "tg_l2_error": r["l2"]}), flush=True)
 
 
+def _timed_run(make, steps: int, warmup: int, graph: bool) -> dict:
+    """Steps of a ready Simulation bracketed by HIP events on its stream
+    (graph replay or eager calls), then a short eager run with per-kernel
+    events for the breakdown (after the state is downloaded)."""
+    sim = make()
+    stepper = sim.step if graph else sim.eager_step
+    stepper(warmup)
+    sim.synchronize()
+    sim.reset_stats()
+    e0, e1 = HipEvent(), HipEvent()
+    e0.record(sim.stream)
+    stepper(steps)
+    e1.record(sim.stream)
+    ms = e0.elapsed_ms(e1)
+    stats = sim.stats()
+    out = sim.download()
+    k = max(1, min(steps, 20))
+    sim.kernel_times()
+    sim.profile(True)
+    sim.eager_step(k)
+    sim.profile(False)
+    kt = sim.kernel_times()
+    sim.close()
+    return {"ms": ms, "stats": stats, "state": out,
+            "breakdown": {g: round(getattr(kt, g + "_ms") / k, 4) for g in ("lsa", "pst", "cells", "ghost", "other")}}
+
+
+def ensemble_bench(args) -> None:
+    """64 independent lid-driven cavities in one handle (every launch covers
+    all of them), RK2 + PST, mixed precision, one HIP-graph replay per step,
+    with the launch settings picked by lssph.tune.autotune on this workload."""
+    from lssph import tune
+    from lssph.engine import Simulation
+    params, states, meta = workloads.cavity_ensemble(count=64, precision=1)
+    ni = [st.num_int for st in states]
+    ne = [st.num_ext for st in states]
+
+    def make(skin: float = 0.08):
+        sim = Simulation(params, ni, ne, precision=1, skin=skin)
+        sim.upload(states)
+        sim.init_cell()
+        sim.ghost()
+        return sim
+
+    print(f"[bench] ensemble: {len(states)} cavities, {sum(ni)} fluid + {sum(ne)} wall particles; autotuning",
+          flush=True)
+    tuned = tune.autotune(make, steps=5)
+    b = tuned.best
+
+    def make_best():
+        sim = make(b.skin)
+        sim.set_tuning(block=b.block, list_len=b.list_len)
+        return sim
+
+    r = _timed_run(make_best, args.steps, args.warmup, graph=True)
+    sizes = {}
+    for m in meta:
+        sizes[f"{m['n']}x{m['n']}"] = sizes.get(f"{m['n']}x{m['n']}", 0) + 1
+    print(json.dumps({
+        "metric": "ensemble particle-steps/s (fluid particles, all instances)",
+        "value": round(sum(ni) * args.steps / (r["ms"] / 1e3), 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["ms"] / args.steps, 4),
+        "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "64 lid-driven cavities, sizes drawn from {32,64,128,256}^2 and Re log-uniform "
+                               "in [100, 1000] (SplitMix64 seed 2030), 3 wall layers, lid velocity 1, WALL_MODEL 2, "
+                               "LS q=5, RK2 + PST every step, mixed precision, one graph replay per step",
+                   "instances": len(states), "sizes": sizes, "fluid_particles": sum(ni), "wall_particles": sum(ne)},
+        "tuning": tuned.as_dict(),
+        "ms_per_step_by_kernel_eager": r["breakdown"],
+        "stats": {"fp64_fallbacks": int(r["stats"].fp64_fallbacks),
+                  "list_overflows": int(r["stats"].list_overflows)}}), flush=True)
+
+
+def boussinesq_bench(args) -> None:
+    """Bottom-heated convection, 1024 x 1024 fluid particles with no-slip
+    walls, 4 coupled fields, WALL_MODEL 3 (6 x 6 truncated supports), RK4 +
+    PST: the mixed path against the FP64 path on the same input, per field,
+    with the mixed path's FP64 re-solves counted."""
+    from lssph.particles import splitmix64_uniform
+    ra = (1e4, 1e5, 1e6)[min(int(splitmix64_uniform(2031, 1)[0] * 3), 2)]
+    n = args.n if args.n != 2048 else 1024
+    runs = {}
+    for prec in (1, 0):
+        param, st = workloads.boussinesq(n=n, ra=ra, jitter=0.2, seed=2030, precision=prec)
+
+        def make(param=param, st=st, prec=prec):
+            sim = simulation_from_state(param, st, precision=prec)
+            sim.ghost()
+            return sim
+
+        print(f"[bench] boussinesq {n}x{n} Ra={ra:g} precision={'mixed' if prec else 'fp64'}", flush=True)
+        runs[prec] = _timed_run(make, args.steps, args.warmup, graph=True)
+    n_part = st.num_int
+    m, f = runs[1], runs[0]
+    print(json.dumps({
+        "metric": "particle-steps/s, Boussinesq convection, mixed precision",
+        "value": round(n_part * args.steps / (m["ms"] / 1e3), 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(m["ms"] / args.steps, 4),
+        "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"bottom-heated Boussinesq convection {n}x{n}, jitter 0.2dx, Ra {ra:g}, Pr 0.71, "
+                               "no-slip walls, T_bottom 1, T_top 0, adiabatic sides, WALL_MODEL 3, LS q=5, "
+                               "RK4 + PST every step",
+                   "particles": n_part, "walls": st.num_ext},
+        "fp64": {"value": round(n_part * args.steps / (f["ms"] / 1e3), 1),
+                 "ms_per_step": round(f["ms"] / args.steps, 4)},
+        "mixed_vs_fp64": field_diff(m["state"], f["state"], n_part,
+                                    fields=("x", "y", "u", "v", "rho", "pre", "tem"), periodic=False),
+        "ms_per_step_by_kernel_eager": m["breakdown"],
+        "stats": {"fp64_fallbacks": int(m["stats"].fp64_fallbacks),
+                  "list_overflows": int(m["stats"].list_overflows)}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +334,9 @@ def main():
                     help="only run 3 device-to-device copies of BYTES (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--sph-model", type=int, default=3,
                     help="1-2 classical SPH, 4-6 LS q=9/14/20: time that operator (FP64) instead of the headline")
+    ap.add_argument("--workload", choices=("taylor-green", "ensemble", "boussinesq"), default="taylor-green",
+                    help="ensemble: 64 lid-driven cavities (autotuned, graph-replayed); boussinesq: 1024^2 "
+                         "convection, mixed vs FP64; each prints a line of its own, not the headline")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -231,6 +346,12 @@ def main():
               "(a domain decomposition is designed but not built); run with --gpus 1.")
         sys.exit(2)
 
+    if args.workload == "ensemble":
+        ensemble_bench(args)
+        return
+    if args.workload == "boussinesq":
+        boussinesq_bench(args)
+        return
     if args.sph_model != 3:
         model_bench(args)
         return

@@ -1,22 +1,22 @@
 // boundary.hip -- ghost-wall particles for gfx950 (ghost_mod.f90:43-78).
 //
-// A group of kGhostLanes lanes per virtual marker (the mirror of a wall
-// particle across the wall): interpolate the fluid values at the marker from
-// its fluid neighbors in the cell-sorted mirror (LS type B with q = 3 or 6,
-// or CSPH), then extrapolate them to the paired wall particle per wall type
-// and boundary condition.  Two lanes per marker halve each lane's serial
-// chain of dependent loads; more lanes lose to the group sums and the
-// repeated solves (100K-marker ensemble: 1, 2, 4, 8 lanes take 69, 49, 51,
-// 58 us per launch).  The wall values are written to the wall particle's
+// A group of lanes per virtual marker (the mirror of a wall particle across
+// the wall): interpolate the fluid values at the marker from its fluid
+// neighbors in the cell-sorted mirror (LS type B with q = 3 or 6, or CSPH),
+// then extrapolate them to the paired wall particle per wall type and
+// boundary condition.  The wall values are written to the wall particle's
 // slot of the current records (read by the next pair kernel).
+//
+// Several lanes per marker shorten each lane's serial chain of dependent
+// loads and put more waves in flight when the markers are few; too many lose
+// to the group sums and the repeated solves.  The launcher takes 2 lanes
+// from 64 K markers up (100 K markers of a 64-cavity ensemble, WALL_MODEL 2:
+// 1, 2, 4, 8 lanes take 69, 49, 51, 58 us per launch) and 4 below (16 K
+// markers of 1024^2 Boussinesq convection, WALL_MODEL 3: 2, 4, 8 lanes take
+// 51, 31, 35 us).
 #include "pair_common.hpp"
 
 namespace lssph {
-
-#ifndef LSSPH_GHOST_LANES
-#define LSSPH_GHOST_LANES 2
-#endif
-constexpr int kGhostLanes = LSSPH_GHOST_LANES;
 
 struct GhostArgs {
   Rec *__restrict__ mir;   // current records: fluid read, wall slots written
@@ -311,6 +311,17 @@ __global__ void k_vm_init(int n_int, int n_ext, const Rec *__restrict__ rec, con
 
 static inline int blocks(int n, int b) { return (n + b - 1) / b; }
 
+template <int G>
+static void ghost_launch(hipStream_t s, int wall_model, int n_ext, const GhostArgs &a) {
+  const dim3 grid(blocks(n_ext, 256 / G)), block(256);
+  if (wall_model == 1)
+    hipLaunchKernelGGL((k_ghost<G, 1>), grid, block, 0, s, a);
+  else if (wall_model == 3)
+    hipLaunchKernelGGL((k_ghost<G, 3>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_ghost<G, 2>), grid, block, 0, s, a);
+}
+
 void launch_ghost(hipStream_t s, Device &d, int cur) {
   if (d.n_ext == 0) return;
   GhostArgs a;
@@ -333,14 +344,10 @@ void launch_ghost(hipStream_t s, Device &d, int cur) {
   a.vm_tem = d.vm_tem;
   a.opt = d.opt;
   a.n_ext = d.n_ext;
-  constexpr int G = kGhostLanes;
-  const dim3 grid(blocks(d.n_ext, 256 / G)), block(256);
-  if (d.opt.wall_model == 1)
-    hipLaunchKernelGGL((k_ghost<G, 1>), grid, block, 0, s, a);
-  else if (d.opt.wall_model == 3)
-    hipLaunchKernelGGL((k_ghost<G, 3>), grid, block, 0, s, a);
+  if (d.n_ext >= 65536)
+    ghost_launch<2>(s, d.opt.wall_model, d.n_ext, a);
   else
-    hipLaunchKernelGGL((k_ghost<G, 2>), grid, block, 0, s, a);
+    ghost_launch<4>(s, d.opt.wall_model, d.n_ext, a);
 }
 
 void launch_vm_init(hipStream_t s, Device &d, int cur) {

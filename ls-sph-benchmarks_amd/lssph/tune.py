@@ -1,0 +1,91 @@
+"""Autotuner of the pair kernels' launch settings for one workload.
+
+The settings (lssph_set_tuning, lssph_set_skin) change which candidates are
+tested and how the work is laid out on the GPU, never the neighbor sets, so
+the results of every candidate setting are the same; only the time differs.
+`autotune` times a few graph-replayed steps per candidate on the caller's own
+workload and returns the fastest.  No reference counterpart: the reference
+has no launch settings.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Sequence
+
+from .engine import LssphError, Simulation
+
+
+@dataclass
+class Setting:
+    block: int        # threads per block of the pair kernels
+    list_len: int     # on-chip neighbor-list entries (0: the size lssph_create chose)
+    skin: float       # cell-list skin as a fraction of h_eff
+    ms_per_step: float = float("nan")
+    error: str = ""
+
+
+@dataclass
+class TuneResult:
+    best: Setting
+    default: Setting
+    table: list = field(default_factory=list)
+
+    def as_dict(self) -> dict:
+        def d(s: Setting) -> dict:
+            out = {"block": s.block, "list_len": s.list_len or "auto", "skin": s.skin,
+                   "ms_per_step": round(s.ms_per_step, 4)}
+            if s.error:
+                out["error"] = s.error
+            return out
+        return {"best": d(self.best), "default": d(self.default), "candidates": [d(s) for s in self.table]}
+
+
+def _time_steps(sim: Simulation, steps: int) -> float:
+    sim.step(1)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def autotune(make_sim: Callable[[float], Simulation], steps: int = 5,
+             blocks: Sequence[int] = (64, 128, 256), list_lens: Sequence[int] = (0,),
+             skins: Sequence[float] = (0.04, 0.08, 0.16),
+             default: tuple = (128, 0, 0.08)) -> TuneResult:
+    """make_sim(skin) returns a fresh Simulation with that skin, uploaded and
+    ready to step (the skin is fixed before the upload).  Every combination
+    of blocks x list_lens x skins is timed over `steps` graph-replayed steps
+    after one warm-up step; a combination the library rejects (an LDS limit,
+    or a skin too narrow for the motion per step) is recorded with its error
+    and skipped."""
+    table = []
+    # list length 0 (the size lssph_create chose) must come first on each
+    # fresh handle: lssph_set_tuning's 0 keeps the current length
+    lens = sorted(set(list_lens), key=lambda n: (n != 0, n))
+    for skin in skins:
+        sim = make_sim(skin)
+        try:
+            for list_len, block in itertools.product(lens, blocks):
+                s = Setting(block, list_len, skin)
+                table.append(s)
+                try:
+                    sim.set_tuning(block=block, list_len=list_len)
+                except LssphError as e:  # e.g. list_len x block beyond the LDS of a block
+                    s.error = str(e)
+                    continue
+                try:
+                    s.ms_per_step = _time_steps(sim, steps)
+                except LssphError as e:  # e.g. the skin guard: this handle is done
+                    s.error = str(e)
+                    break
+        finally:
+            sim.close()
+    ok = [s for s in table if not s.error]
+    if not ok:
+        raise RuntimeError("autotune: no candidate setting ran")
+    best = min(ok, key=lambda s: s.ms_per_step)
+    dflt = next((s for s in ok if (s.block, s.list_len, s.skin) == tuple(default)), best)
+    return TuneResult(best, dflt, table)

@@ -166,6 +166,31 @@ def test_cavity_ensemble_matches_oracle_per_instance():
         assert max(e.values()) <= TOL, e
 
 
+def test_large_ensemble_ghost_path_matches_oracle():
+    """22 cavities of 256^2 carry 68 K virtual markers, which the ghost
+    launcher serves with 2 lanes per marker (4 below 64 K, as in the smaller
+    cases); the first and last instance against the oracle."""
+    params, states, meta = workloads.cavity_ensemble(count=22, sizes=(256,), seed=8, precision=0)
+    ni = [s.num_int for s in states]
+    ne = [s.num_ext for s in states]
+    assert sum(ne) >= 65536
+    print(f"[gpu] large ensemble: {len(states)} cavities, {sum(ne)} markers", flush=True)
+    with Simulation(params, ni, ne, precision=0) as sim:
+        sim.upload(states)
+        sim.init_cell()
+        sim.ghost()
+        sim.step(2)
+        out = split_states(sim.download(), ni, ne)
+    for k in (0, len(states) - 1):
+        p, st = params[k], states[k]
+        o = OracleSim(p.to_c(), st.num_int, st.num_ext)
+        o.init(st)
+        o.ghost()
+        o.step(2)
+        e = _cmp(out[k], o.get_state(), ("x", "y", "u", "v", "rho", "pre"))
+        assert max(e.values()) <= TOL, (k, e)
+
+
 def test_boussinesq_wall_model3_matches_oracle():
     param, st = workloads.boussinesq(n=24, ra=1e4, jitter=0.2, seed=3)
     o, s = _pair(param, st)
