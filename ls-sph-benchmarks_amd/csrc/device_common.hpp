@@ -15,12 +15,15 @@ namespace lssph {
 
 constexpr double kPi = 3.141592653589793;  // acos(-1.0d0)
 
-// One particle; 64 B = two dwordx4 pairs.  vol = sp_mass / rho is kept next
-// to rho so pair loops never divide (the quotient is the same IEEE value the
-// reference forms per pair, LSSPH_A_mod.f90:136).
+// One particle; 64 B = four 16-B chunks.  vol = sp_mass / rho is kept so
+// pair loops never divide (the quotient is the same IEEE value the reference
+// forms per pair, LSSPH_A_mod.f90:136).  The order puts what one neighbor
+// gather needs in as few chunks as possible: (x, y) and (u, v); rho and vol
+// side by side for the shifting (rho differences, volumes), vol and pre side
+// by side for the LS operator (one 16-B load at byte 40).
 struct alignas(64) Rec {
   double x, y, u, v;
-  double rho, pre, vol, tem;
+  double rho, vol, pre, tem;
 };
 
 // Packed per-particle info: ptype in bits 0..7 (0 = fluid), instance id in
