@@ -63,11 +63,11 @@ struct LsSums {
       for (int r = 0; r <= i; ++r) M[i * (i + 1) / 2 + r] += vwa * a[r];
     }
   }
-  // Mixed path, q = 5: with the basis [x, y, x^2/2, y^2/2, xy] the moment
-  // matrix has 12 distinct sums, M(2,1) = M(4,0)/2, M(3,0) = M(4,1)/2 and
-  // M(3,2) = M(4,4)/4, so add_reduced skips those three and complete()
-  // derives them after the loop (exact scalings; the results differ from
-  // direct accumulation by FP32 rounding of the sums only).
+  // Mixed path, q = 5, with the basis of basis5s, [x, y, x^2, y^2, xy]: the
+  // moment matrix has 12 distinct sums, M(2,1) = M(4,0), M(3,0) = M(4,1) and
+  // M(3,2) = M(4,4), so add_reduced skips those three and complete() copies
+  // them after the loop (the results differ from direct accumulation by
+  // FP32 rounding of the sums only).
   __device__ __forceinline__ void add_reduced(R vw, const R (&a)[Q], const R (&df)[NF]) {
     static_assert(Q == 5, "q = 5 basis");
 #pragma unroll
@@ -83,9 +83,9 @@ struct LsSums {
     }
   }
   __device__ __forceinline__ void complete() {
-    M[4] = R(0.5) * M[10];   // (2,1) from (4,0)
-    M[6] = R(0.5) * M[11];   // (3,0) from (4,1)
-    M[8] = R(0.25) * M[14];  // (3,2) from (4,4)
+    M[4] = M[10];  // (2,1) from (4,0)
+    M[6] = M[11];  // (3,0) from (4,1)
+    M[8] = M[14];  // (3,2) from (4,4)
   }
 };
 
@@ -131,13 +131,29 @@ __device__ __forceinline__ void basis_q(double x, double y, const InstDev &I, do
   }
 }
 
-// Same basis from offsets already scaled by 1/h (mixed precision).
+// The basis from offsets already scaled by 1/h (mixed precision), without
+// the factors 1/2 of the squares: [x, y, x^2, y^2, xy].  Every sum of M and
+// b is then the one of basis5 times an exact power of two (S M S and S b,
+// S = diag(1, 1, 2, 2, 1)), and LDL^T without pivoting commutes with such a
+// scaling, so ldlt_solve_refine returns bitwise the coefficients of the
+// halved basis after multiplying entries 2 and 3 by 2; two multiplications
+// per neighbor are saved.
 __device__ __forceinline__ void basis5s(float xs, float ys, float (&a)[5]) {
   a[0] = xs;
   a[1] = ys;
-  a[2] = 0.5f * (xs * xs);
-  a[3] = 0.5f * (ys * ys);
+  a[2] = xs * xs;
+  a[3] = ys * ys;
   a[4] = xs * ys;
+}
+
+// Coefficients of the basis5s sums -> coefficients of the basis5 terms.
+template <int NF>
+__device__ __forceinline__ void unscale_squares(double (&X)[NF][5]) {
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    X[f][2] *= 2.0;
+    X[f][3] *= 2.0;
+  }
 }
 
 template <typename R, int NF, int Q>
@@ -161,14 +177,15 @@ __device__ __forceinline__ double sym5(const R (&M)[15], int i, int c) {
   return i >= c ? double(M[i * (i + 1) / 2 + c]) : double(M[c * (c + 1) / 2 + i]);
 }
 
-// FP64 LDL^T of the symmetric 5x5 moment matrix held as FP32 sums, with one
-// step of iterative refinement per right-hand side (x += M^-1 (b - M x),
-// the residual formed in FP64 against the FP32 sums).  The right-hand sides
-// are processed one at a time and the sums stay FP32 in registers, which
-// keeps the solve's register footprint near that of the pair loop.  Returns
-// false when a pivot is not positive or the pivot ratio max(d)/min(d) -- a
-// cheap lower bound of cond_2(M) -- exceeds cond_max; the caller then
-// re-solves in full FP64.
+// FP64 LDL^T of the symmetric 5x5 moment matrix held as FP32 sums (of the
+// basis5s basis), with one step of iterative refinement per right-hand side
+// (x += M^-1 (b - M x), the residual formed in FP64 against the FP32 sums).
+// The right-hand sides are processed one at a time and the sums stay FP32 in
+// registers, which keeps the solve's register footprint near that of the
+// pair loop.  Returns false when a pivot is not positive or the pivot ratio
+// max(d)/min(d) of the basis5 system -- a cheap lower bound of its
+// cond_2(M) -- exceeds cond_max; the caller then re-solves in full FP64.
+// X holds the coefficients of the basis5 terms.
 template <int NF>
 __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, double (&X)[NF][5],
                                                   double cond_max) {
@@ -183,8 +200,10 @@ __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, do
     ok = ok && (dj > 0.0);
     d[j] = dj;
     dinv[j] = 1.0 / dj;
-    dmax = fmax(dmax, dj);
-    dmin = fmin(dmin, dj);
+    // pivots of the basis5 system: those of the squares are 4x larger here
+    const double dj5 = (j == 2 || j == 3) ? 0.25 * dj : dj;
+    dmax = fmax(dmax, dj5);
+    dmin = fmin(dmin, dj5);
 #pragma unroll
     for (int i = j + 1; i < 5; ++i) {
       double t = sym5(S.M, i, j);
@@ -223,6 +242,7 @@ __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, do
 #pragma unroll
     for (int i = 0; i < 5; ++i) X[f][i] = x[i] + r[i];
   }
+  unscale_squares<NF>(X);
   return true;
 }
 

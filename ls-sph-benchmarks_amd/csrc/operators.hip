@@ -288,6 +288,7 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
       double M[Q][Q];
       unpack(S, M, X);
       lu_solve<Q, NF>(M, X);
+      if constexpr (MIXED) unscale_squares<NF>(X);  // the mixed sums are of the basis5s basis
     }
   } else {
 #pragma unroll
