@@ -31,6 +31,8 @@ struct NbList {
   int list_len;  // entries before the trash entry
   int cnt;       // entries, the particle itself included; > list_len: overflowed
   int base;      // >= 0: 16-bit entries relative to base; -1: 32-bit entries
+  bool pos;      // 16-bit entries are row << 14 | staging index: positions stay in LDS
+  unsigned long long rowd;  // with pos: slot = base + index + 16-bit field `row` of rowd
 };
 
 __device__ __forceinline__ NbList make_list(int32_t *lds, int list_len, int tid) {
@@ -40,6 +42,8 @@ __device__ __forceinline__ NbList make_list(int32_t *lds, int list_len, int tid)
   L.list_len = list_len;
   L.cnt = 0;
   L.base = -1;
+  L.pos = false;
+  L.rowd = 0;
   return L;
 }
 
@@ -51,6 +55,8 @@ __device__ __forceinline__ NbList no_list() {
   L.list_len = 0;
   L.cnt = 1 << 30;
   L.base = -1;
+  L.pos = false;
+  L.rowd = 0;
   return L;
 }
 
@@ -263,8 +269,12 @@ __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const i
   const int cap = 8 * (L.list_len + 1);  // 16-byte positions in the upper half of the wave's rows
   const int base = cstart[off + y0 * nx + x0];
   if (cstart[off + y1 * nx + x1 + 1] - base > 0xffff) return false;
-  for (int yy = y0; yy <= y1; ++yy)
-    if (cstart[off + yy * nx + x1 + 1] - cstart[off + yy * nx + x0] > cap) return false;
+  int total = 0;
+  for (int yy = y0; yy <= y1; ++yy) {
+    const int n_row = cstart[off + yy * nx + x1 + 1] - cstart[off + yy * nx + x0];
+    if (n_row > cap) return false;
+    total += n_row;
+  }
 
   const int lane = L.lane;
   char *l16 = L.wbase + 2 * lane;
@@ -272,30 +282,64 @@ __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const i
   const unsigned list_len = (unsigned)L.list_len;
   const int lx0 = max(cx - 1, 0), lx1 = min(cx + 1, nx - 1);
   int cnt = 0;
-  auto accept = [&](int t, double2 p) {
-    const unsigned k = min((unsigned)cnt, list_len);
-    *reinterpret_cast<uint16_t *>(l16 + (k << 7)) = (uint16_t)(t - base);
-    cnt += (sq_dist(p.x - xi, p.y - yi) <= r2) ? 1 : 0;
-  };
-#pragma unroll 1
-  for (int yy = y0; yy <= y1; ++yy) {
-    const int row = off + yy * nx;
-    const int u0 = cstart[row + x0], u1 = cstart[row + x1 + 1];
-    const int a0 = cstart[row + lx0];
-    const int a1 = (yy >= cy - 1 && yy <= cy + 1) ? cstart[row + lx1 + 1] : a0;
-    wave_sync();  // the previous row's reads of the staging area come first
-    for (int i = lane; i < u1 - u0; i += 64) stage[i] = load_xy(mir, u0 + i);
-    wave_sync();
+  auto scan_window = [&](int a0, int a1, int shift, auto &&entry) {
+    auto accept = [&](int t, double2 p) {
+      const unsigned k = min((unsigned)cnt, list_len);
+      *reinterpret_cast<uint16_t *>(l16 + (k << 7)) = entry(t);
+      cnt += (sq_dist(p.x - xi, p.y - yi) <= r2) ? 1 : 0;
+    };
     int t = a0;
     for (; t + 4 <= a1; t += 4) {
-      const double2 *sp = stage + (t - u0);
+      const double2 *sp = stage + (t - shift);
       const double2 p0 = sp[0], p1 = sp[1], p2 = sp[2], p3 = sp[3];
       accept(t, p0);
       accept(t + 1, p1);
       accept(t + 2, p2);
       accept(t + 3, p3);
     }
-    for (; t < a1; ++t) accept(t, stage[t - u0]);
+    for (; t < a1; ++t) accept(t, stage[t - shift]);
+  };
+  if (total <= cap) {
+    // every row at once: the positions stay in LDS for the walk, and a list
+    // entry is row << 14 | staging index (rows 0..3, index < cap <= 1032)
+    unsigned long long rowd = 0;
+    int seg = 0;
+    for (int yy = y0, r = 0; yy <= y1; ++yy, ++r) {
+      const int row = off + yy * nx;
+      const int u0 = cstart[row + x0], n_row = cstart[row + x1 + 1] - u0;
+      for (int i = lane; i < n_row; i += 64) stage[seg + i] = load_xy(mir, u0 + i);
+      // rows are disjoint ascending slot ranges: 0 <= (u0 - base) - seg <= 0xffff
+      rowd |= (unsigned long long)((u0 - base) - seg) << (16 * r);
+      seg += n_row;
+    }
+    wave_sync();
+    seg = 0;
+#pragma unroll 1
+    for (int yy = y0, r = 0; yy <= y1; ++yy, ++r) {
+      const int row = off + yy * nx;
+      const int u0 = cstart[row + x0], n_row = cstart[row + x1 + 1] - u0;
+      const int a0 = cstart[row + lx0];
+      const int a1 = (yy >= cy - 1 && yy <= cy + 1) ? cstart[row + lx1 + 1] : a0;
+      const int shift = u0 - seg, tag = (r << 14) - shift;
+      scan_window(a0, a1, shift, [&](int t) { return (uint16_t)(t + tag); });
+      seg += n_row;
+    }
+    L.pos = true;
+    L.rowd = rowd;
+  } else {
+    // one row at a time; a list entry is slot - base
+#pragma unroll 1
+    for (int yy = y0; yy <= y1; ++yy) {
+      const int row = off + yy * nx;
+      const int u0 = cstart[row + x0], u1 = cstart[row + x1 + 1];
+      const int a0 = cstart[row + lx0];
+      const int a1 = (yy >= cy - 1 && yy <= cy + 1) ? cstart[row + lx1 + 1] : a0;
+      wave_sync();  // the previous row's reads of the staging area come first
+      for (int i = lane; i < u1 - u0; i += 64) stage[i] = load_xy(mir, u0 + i);
+      wave_sync();
+      scan_window(a0, a1, u0, [&](int t) { return (uint16_t)(t - base); });
+    }
+    L.pos = false;
   }
   L.cnt = cnt;
   L.base = base;
@@ -344,7 +388,20 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
     walk_stencil(mir, cstart, I, lc, s, me, f);
     return;
   }
-  if (L.base >= 0) {
+  if (L.base >= 0 && L.pos) {
+    // positions from the staging area, the other fields from the records
+    const char *l16 = L.wbase + 2 * L.lane;
+    const double2 *stage = reinterpret_cast<const double2 *>(L.wbase + 128 * (L.list_len + 1));
+    for (int k = 0; k < cnt; ++k) {
+      const unsigned e = *reinterpret_cast<const uint16_t *>(l16 + (k << 7));
+      const unsigned i = e & 0x3fffu;
+      // (e >> 14) * 16 = (e >> 10) & 0x30, as i < 1024 + 8
+      const unsigned d = (unsigned)(L.rowd >> ((e >> 10) & 0x30u)) & 0xffffu;
+      const double2 p = stage[i];
+      const Rec nb = mir[L.base + (int)(i + d)];
+      f(nb, p.x - me.x, p.y - me.y);
+    }
+  } else if (L.base >= 0) {
     const char *l16 = L.wbase + 2 * L.lane;
     for (int k = 0; k < cnt; ++k) {
       const Rec nb = mir[L.base + *reinterpret_cast<const uint16_t *>(l16 + (k << 7))];
