@@ -323,7 +323,7 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
   const int kt = A.kt;
   LsSums<float, NF> S;
   S.zero();
-  walk_neighbors(A.mir, A.cstart, I, lc, s, me, L,
+  walk_neighbors<true>(A.mir, A.cstart, I, lc, s, me, L,
                  [&](const Rec &nb, double dx, double dy) {
                    const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
                    const float q = fast_sqrtf(xs * xs + ys * ys);

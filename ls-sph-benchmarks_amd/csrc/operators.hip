@@ -213,7 +213,9 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
   S.zero();
   R sx = R(0), sy = R(0);
   const double inv_w_ave = 1.0 / I.w_ave;
-  walk_neighbors(A.mir, A.cstart, I, lc, s, me, nl,
+  // the deep walk for the mixed Wendland kernel without temperature (the
+  // others would spill under the 128-register cap)
+  walk_neighbors<MIXED && !T && KT == 3>(A.mir, A.cstart, I, lc, s, me, nl,
                  [&](const Rec &nb, double dx, double dy) {
                    R df[NF];
                    df[0] = R(nb.u - me.u);
@@ -321,10 +323,11 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
 }
 
 
-// The mixed kernel is held to 128 VGPRs (4 waves per SIMD); its two
-// instance paths otherwise need 129.
+// The mixed kernel without temperature is held to 128 VGPRs (4 waves per
+// SIMD); its two instance paths otherwise need 129.  With temperature the
+// cap would spill ~50 registers, so that one keeps 3 waves.
 template <bool T, bool MIXED, int KT, int Q>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 4 : 1))) k_pst(PstArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !T ? 4 : 1))) k_pst(PstArgs A) {
   static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
   extern __shared__ int32_t lds[];
   const int tid = threadIdx.x, B = blockDim.x;

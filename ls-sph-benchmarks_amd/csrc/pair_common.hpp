@@ -375,11 +375,26 @@ __device__ __forceinline__ void walk_stencil(const Rec *__restrict__ mir, const 
   });
 }
 
+// Loop over k = 0 .. n-1 calling body(k); with DEEP the loop is unrolled
+// four times so that four neighbors' loads are in flight before their sums
+// (still added in list order), else the compiler picks the unrolling.  The
+// walk waits on gathers more than on arithmetic, but the deeper loop needs
+// registers that the heavier kernels do not have.
+template <bool DEEP, typename B>
+__device__ __forceinline__ void list_loop(int n, B &&body) {
+  if constexpr (DEEP) {
+#pragma unroll 4
+    for (int k = 0; k < n; ++k) body(k);
+  } else {
+    for (int k = 0; k < n; ++k) body(k);
+  }
+}
+
 // Calls f(nb_record, dx, dy) for every accepted neighbor of slot s in
-// summation order: from the LDS list filled by scan_candidates, which holds
-// the particle itself as a pair with dx = dy = 0, or (the list overflowed)
-// by a walk over the candidates, which skips it.
-template <typename F>
+// summation order: from the LDS list filled by scan_candidates or
+// scan_staged, which holds the particle itself as a pair with dx = dy = 0,
+// or (the list overflowed) by a walk over the candidates, which skips it.
+template <bool DEEP = false, typename F>
 __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
                                                const InstDev &I, int lc, int s, const Rec &me, const NbList &L,
                                                F &&f) {
@@ -392,7 +407,7 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
     // positions from the staging area, the other fields from the records
     const char *l16 = L.wbase + 2 * L.lane;
     const double2 *stage = reinterpret_cast<const double2 *>(L.wbase + 128 * (L.list_len + 1));
-    for (int k = 0; k < cnt; ++k) {
+    list_loop<DEEP>(cnt, [&](int k) {
       const unsigned e = *reinterpret_cast<const uint16_t *>(l16 + (k << 7));
       const unsigned i = e & 0x3fffu;
       // (e >> 14) * 16 = (e >> 10) & 0x30, as i < 1024 + 8
@@ -400,13 +415,13 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
       const double2 p = stage[i];
       const Rec nb = mir[L.base + (int)(i + d)];
       f(nb, p.x - me.x, p.y - me.y);
-    }
+    });
   } else if (L.base >= 0) {
     const char *l16 = L.wbase + 2 * L.lane;
-    for (int k = 0; k < cnt; ++k) {
+    list_loop<DEEP>(cnt, [&](int k) {
       const Rec nb = mir[L.base + *reinterpret_cast<const uint16_t *>(l16 + (k << 7))];
       f(nb, nb.x - me.x, nb.y - me.y);
-    }
+    });
   } else if (!wave_wraps(I, lc)) {
     const char *l32 = L.wbase + 4 * L.lane;
     for (int k = 0; k < cnt; ++k) {
