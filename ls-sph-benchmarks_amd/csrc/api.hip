@@ -476,7 +476,7 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   {
     int64_t nmax = 1;
     for (int k = 0; k < n_inst; ++k) nmax = std::max<int64_t>(nmax, (int64_t)n_int[k] + n_ext[k]);
-    d.red_blocks = (int)std::min<int64_t>(256, (nmax + 4095) / 4096);
+    d.red_blocks = (int)std::min<int64_t>(1024, (nmax + 4095) / 4096);
   }
   {
     // on-chip list length from the expected neighbor count pi (h_eff/dx)^2

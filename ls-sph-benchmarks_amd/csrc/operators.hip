@@ -386,9 +386,13 @@ __global__ void __launch_bounds__(256) k_umax_spd(const double *__restrict__ spd
                                                   const InstDev *__restrict__ inst, unsigned long long *umax) {
   const InstDev &I = inst[blockIdx.y];
   const int t0 = cstart[I.cell_off], t1 = cstart[I.cell_off + I.nx * I.ny];
+  const int stride = gridDim.x * blockDim.x;
   double s = 0.0;
-  for (int t = t0 + blockIdx.x * blockDim.x + threadIdx.x; t < t1; t += gridDim.x * blockDim.x)
-    s = fmax(s, spd[t]);
+  int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+  // four independent loads per iteration (a maximum is exact in any order)
+  for (; t + 3 * stride < t1; t += 4 * stride)
+    s = fmax(fmax(s, spd[t]), fmax(fmax(spd[t + stride], spd[t + 2 * stride]), spd[t + 3 * stride]));
+  for (; t < t1; t += stride) s = fmax(s, spd[t]);
   const double m = block_max(s);
   if (threadIdx.x == 0) atomic_max_nonneg(&umax[blockIdx.y], m);
 }
