@@ -145,6 +145,33 @@ def test_mixed_precision_close_to_fp64():
             assert rel_err(b[k], a[k]) < 1e-4, k
 
 
+def test_mixed_fp64_fallback_path_matches_fp64():
+    """A condition limit below every pivot ratio sends every mixed solve to
+    its FP64 fallback: k_lsa's list-driven FP64 relaunch and the shifting's
+    in-kernel LU on the FP32 sums (coefficients of the squares rescaled).
+    The fallbacks are counted, and the velocities agree with the FP64 path
+    at least as closely as the mixed path's (the RHS passes are FP64 there;
+    the positions, which the FP32 shift sums set, agree equally)."""
+    from lssph import config as lcfg
+    param, st = workloads.taylor_green_periodic(n=64, re=1000.0, rk=4, jitter=0.2, seed=14)
+    param_low = lcfg.init_param(param.cfg.replace(cond_max=1.0))
+    with simulation_from_state(param, st, precision=0) as f64, \
+            simulation_from_state(param_low, st, precision=1) as fb, \
+            simulation_from_state(param, st, precision=1) as mix:
+        for s in (f64, fb, mix):
+            s.step(3)
+        a, b, c = f64.download(), fb.download(), mix.download()
+        assert fb.stats().fp64_fallbacks >= 4 * 3 * st.num_int
+        assert mix.stats().fp64_fallbacks == 0
+        errs = {k: (rel_err(b[k], a[k]), rel_err(c[k], a[k])) for k in ("x", "y", "u", "v", "rho", "pre")}
+        print(f"[gpu] fallback vs fp64, mixed vs fp64: {errs}", flush=True)
+        for k, (e_fb, e_mix) in errs.items():
+            assert e_fb <= 1e-6, (k, e_fb)
+        # the RHS passes are FP64 there; only the shifting sums in FP32
+        for k in ("u", "v"):
+            assert errs[k][0] <= errs[k][1], (k, errs[k])
+
+
 def test_cavity_ensemble_matches_oracle_per_instance():
     params, states, meta = workloads.cavity_ensemble(count=5, sizes=(12, 16, 20), seed=5, precision=0)
     print(f"[gpu] ensemble sizes={[m['n'] for m in meta]}", flush=True)
