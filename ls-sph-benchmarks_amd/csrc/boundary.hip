@@ -49,6 +49,60 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
+// Calls add(nb, x_j - x_m, y_j - y_m, r^2) for the fluid candidates t0,
+// t0 + G, ... below t1 within h_eff of the marker (mx, my), in that order.
+// With BATCH, four candidates at a time: their types and positions are
+// loaded together, then the records of the accepted ones, so a lane waits on
+// two load rounds per four candidates instead of up to two per candidate.
+// That pays for the q = 6 fit (16 K markers of 1024^2 Boussinesq convection:
+// 28.5 instead of 31 us per launch) and not for the lighter q = 3 fit (the
+// 64-cavity ensemble's 100 K markers: 57 instead of 49 us, the batch's
+// registers costing a wave per SIMD).  (The CSPH sums take the offsets'
+// sign of interpolate_CSPH_mod.f90, x_m - x_j: r^2 is the same, and the
+// Shepard sums use r only.)
+template <int G, bool BATCH, typename F>
+__device__ __forceinline__ void fluid_candidates(const GhostArgs &A, const InstDev &I, int t0, int t1, double mx,
+                                                 double my, F &&add) {
+  int t = t0;
+  if constexpr (BATCH) {
+    for (; t + 3 * G < t1; t += 4 * G) {
+      int32_t inf[4];
+      double2 xy[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        inf[j] = A.sinfo[t + j * G];
+        xy[j] = load_xy(A.mir, t + j * G);
+      }
+      double xj[4], yj[4], r2[4];
+      bool ok[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        xj[j] = xy[j].x - mx;
+        yj[j] = xy[j].y - my;
+        r2[j] = sq_dist(xj[j], yj[j]);
+        ok[j] = info_ptype(inf[j]) == 0 && r2[j] <= I.h_eff_sq;
+      }
+      Rec nb[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (ok[j]) nb[j] = A.mir[t + j * G];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (ok[j]) add(nb[j], xj[j], yj[j], r2[j]);
+    }
+  }
+  for (; t < t1; t += G) {
+    const int32_t inf = A.sinfo[t];
+    const double2 xy = load_xy(A.mir, t);
+    if (info_ptype(inf) != 0) continue;
+    const double xj = xy.x - mx, yj = xy.y - my;
+    const double r2 = sq_dist(xj, yj);
+    if (r2 > I.h_eff_sq) continue;
+    const Rec nb = A.mir[t];
+    add(nb, xj, yj, r2);
+  }
+}
+
 // interpolate_LSSPH_B_mod.f90:24-178: a = [1, x/h, y/h (, x^2/2h^2, xy/h^2,
 // y^2/2h^2)], fluid neighbors only, value = constant term of the fit.  The
 // G lanes of a marker take every G-th candidate of each contiguous cell run
@@ -67,37 +121,31 @@ __device__ __forceinline__ void interp_lssph_b(const GhostArgs &A, const InstDev
   const double mx = A.vm_x[k], my = A.vm_y[k];
   const double inv_h = I.inv_h, inv_h2 = I.inv_h_sq;
   int num = 0;
-  visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int, int) {
-    for (int t = t0 + lane; t < t1; t += G) {
-      const int32_t inf = A.sinfo[t];
-      const double2 xy = load_xy(A.mir, t);
-      if (info_ptype(inf) != 0) continue;
-      const double xj = xy.x - mx, yj = xy.y - my;
-      const double r2 = sq_dist(xj, yj);
-      if (r2 > I.h_eff_sq) continue;
-      const Rec nb = A.mir[t];
-      const double w = I.w_norm * kshape_w<double>(kt, sqrt(r2) * I.inv_h);
-      ++num;
-      const double vw = nb.vol * w;
-      double a[Q];
-      a[0] = 1.0;
-      a[1] = xj * inv_h;
-      a[2] = yj * inv_h;
-      if constexpr (Q == 6) {
-        a[3] = xj * xj * 0.5 * inv_h2;
-        a[4] = xj * yj * inv_h2;
-        a[5] = yj * yj * 0.5 * inv_h2;
-      }
-      const double fv[4] = {nb.u, nb.v, nb.pre, has_t ? nb.tem : 0.0};
-#pragma unroll
-      for (int i = 0; i < Q; ++i) {
-        const double vwa = vw * a[i];
-#pragma unroll
-        for (int f = 0; f < 4; ++f) b[f][i] += vwa * fv[f];
-#pragma unroll
-        for (int r = 0; r <= i; ++r) M[i * (i + 1) / 2 + r] += vwa * a[r];
-      }
+  auto add = [&](const Rec &nb, double xj, double yj, double r2) {
+    const double w = I.w_norm * kshape_w<double>(kt, sqrt(r2) * I.inv_h);
+    ++num;
+    const double vw = nb.vol * w;
+    double a[Q];
+    a[0] = 1.0;
+    a[1] = xj * inv_h;
+    a[2] = yj * inv_h;
+    if constexpr (Q == 6) {
+      a[3] = xj * xj * 0.5 * inv_h2;
+      a[4] = xj * yj * inv_h2;
+      a[5] = yj * yj * 0.5 * inv_h2;
     }
+    const double fv[4] = {nb.u, nb.v, nb.pre, has_t ? nb.tem : 0.0};
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const double vwa = vw * a[i];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) b[f][i] += vwa * fv[f];
+#pragma unroll
+      for (int r = 0; r <= i; ++r) M[i * (i + 1) / 2 + r] += vwa * a[r];
+    }
+  };
+  visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int, int) {
+    fluid_candidates<G, Q == 6>(A, I, t0 + lane, t1, mx, my, add);
   });
   num = group_sum<G>(num);
 #pragma unroll
@@ -132,24 +180,18 @@ __device__ __forceinline__ void interp_csph(const GhostArgs &A, const InstDev &I
   const double mx = A.vm_x[k], my = A.vm_y[k];
   double su = 0, sv = 0, sp = 0, st = 0, sw = 0;
   int num = 0;
+  auto add = [&](const Rec &nb, double, double, double r2) {
+    const double w = I.w_norm * kshape_w<double>(kt, sqrt(r2) * I.inv_h);
+    ++num;
+    const double vw = nb.vol * w;
+    sw = sw + vw;
+    su = su + vw * nb.u;
+    sv = sv + vw * nb.v;
+    sp = sp + vw * nb.pre;
+    if (has_t) st = st + vw * nb.tem;
+  };
   visit_cells(I, A.vm_cell[k] - I.cell_off, A.cstart, [&](int t0, int t1, int, int) {
-    for (int t = t0 + lane; t < t1; t += G) {
-      const int32_t inf = A.sinfo[t];
-      const double2 xy = load_xy(A.mir, t);
-      if (info_ptype(inf) != 0) continue;
-      const double xij = mx - xy.x, yij = my - xy.y;
-      const double r2 = sq_dist(xij, yij);
-      if (r2 > I.h_eff_sq) continue;
-      const Rec nb = A.mir[t];
-      const double w = I.w_norm * kshape_w<double>(kt, sqrt(r2) * I.inv_h);
-      ++num;
-      const double vw = nb.vol * w;
-      sw = sw + vw;
-      su = su + vw * nb.u;
-      sv = sv + vw * nb.v;
-      sp = sp + vw * nb.pre;
-      if (has_t) st = st + vw * nb.tem;
-    }
+    fluid_candidates<G, false>(A, I, t0 + lane, t1, mx, my, add);
   });
   num = group_sum<G>(num);
   sw = group_sum<G>(sw);
