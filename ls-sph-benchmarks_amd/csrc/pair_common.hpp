@@ -238,8 +238,11 @@ __device__ __forceinline__ int scan_candidates(const Rec *__restrict__ mir, cons
 // wrap read the union of their candidate rows (three or four contiguous
 // slot ranges) once, coalesced, into LDS, and each lane then tests its own
 // window from there: one 16-byte load per lane per 64 candidates instead of
-// one per lane per candidate.  The accepted entries and their order are those of
-// scan_candidates.  Returns false, with the list untouched, when the wave
+// one per lane per candidate.  When all rows fit the staging area at once,
+// the positions stay there for the walk (list entries row << 14 | staging
+// index, L.pos); otherwise the rows are staged one after another (entries
+// slot - base).  Either way the accepted entries and their order are those
+// of scan_candidates.  Returns false, with the list untouched, when the wave
 // does not qualify (partial or mixed waves, periodic edges, a row union
 // larger than the staging area, a slot span beyond 16 bits).
 __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
