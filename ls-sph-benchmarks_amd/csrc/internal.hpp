@@ -18,7 +18,8 @@ enum {
   kStrayOverflow = 4,
   kSkinExceeded = 5,
   kWaveListOverflow = 6,  // a particle of the q = 20 kernels with too many neighbors
-  kNumCounters = 7
+  kFbDone = 7,            // blocks of k_lsa_fallback done (the last one clears the queue)
+  kNumCounters = 8
 };
 
 // Incremental cell rebuild: particles whose new cell is outside the 3x3

@@ -265,6 +265,7 @@ struct LsaArgs {
   int32_t *counters;
   int32_t *fb_list;
   int32_t *fb_count;
+  int32_t *fb_done;
   MoveOut mv;
   int n, kt, list_len, mode;
   int staged;  // wave-staged candidate scan allowed

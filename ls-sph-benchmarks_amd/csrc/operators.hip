@@ -101,6 +101,17 @@ __global__ void __launch_bounds__(256) k_lsa_fallback(LsaArgs A) {
     atomicAdd(&A.stats64[kFallbacks], 1ull);
     finish<CLS, DELTA>(A, I, A.items[s], s, me, d);
   }
+  // The last block to finish empties the queue for the next launch: every
+  // block has read the count before it reports, so no clearing kernel has
+  // to run before the mixed kernel.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(A.fb_done, 1) == (int)gridDim.x - 1) {
+      *A.fb_count = 0;
+      *A.fb_done = 0;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -470,6 +481,7 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
   a.counters = d.counters;
   a.fb_list = d.fb_list;
   a.fb_count = d.counters + kFbCount;
+  a.fb_done = d.counters + kFbDone;
   a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters, d.xbuild, guard, keys};
   a.n = d.n_total;
   a.kt = d.opt.kernel_type;
@@ -524,7 +536,6 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
     return;
   }
   const bool mixed = d.opt.precision == LSSPH_MIXED;
-  if (mixed) launch_fill_i32(s, a.fb_count, 0, 1);
   if (cls == 0)
     lsa_dispatch<0, false>(s, a, d.block, lds, mixed, kt);
   else if (cls == 1)
