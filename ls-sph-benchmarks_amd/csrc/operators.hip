@@ -64,8 +64,12 @@ __device__ __forceinline__ void lsa_particle(const LsaArgs &A, const InstDev &I,
   finish<CLS, DELTA>(A, I, A.items[s], s, me2, d);
 }
 
+// The FP64 flow kernel without delta-SPH is held to 128 VGPRs, 4 waves per
+// SIMD instead of 3 (2 registers spill; 6 % faster at 2048^2); the others
+// would spill far more.
 template <int CLS, bool MIXED, bool DELTA, int KT, int Q = 5>
-__global__ void __launch_bounds__(256) k_lsa(LsaArgs A) {
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(!MIXED && CLS == 1 && !DELTA && Q == 5 ? 4 : 1))) k_lsa(LsaArgs A) {
   static_assert(!MIXED || Q == 5, "the mixed path is q = 5");
   extern __shared__ int32_t lds[];
   const int tid = threadIdx.x, B = blockDim.x;
