@@ -53,6 +53,28 @@ def test_periodic_heat_matches_oracle():
         assert e <= TOL, e
 
 
+def test_taylor_green_64_full_run_matches_oracle():
+    """The plain-method configuration at full length: 64^2 regular lattice,
+    Re 100, RK2 with shifting, dt = 1e-3 (COE_CFL 0.64, K_REF 100), 100
+    steps, FP64.  The GPU trajectory matches the oracle's, and so does the
+    Taylor-Green L2 velocity error against the analytic decay."""
+    param, st = workloads.taylor_green_periodic(n=64, re=100.0, rk=2, coe_cfl=0.64, jitter=0.0, seed=0)
+    assert abs(param.dt - 1e-3) <= 1e-15, param.dt
+    o, s = _pair(param, st)
+    with s:
+        s.step(100)
+        got = s.download()
+    o.step(100)
+    ref = o.get_state()
+    e = _cmp(got, ref, ("x", "y", "u", "v", "rho", "pre"))
+    print(f"[gpu] TG 64^2, 100 steps: {e}", flush=True)
+    assert max(e.values()) <= 1e-10, e
+    t = 100 * param.dt
+    l2_g = workloads.tg_l2_error(param, got, t, st.num_int)
+    l2_o = workloads.tg_l2_error(param, ref, t, st.num_int)
+    assert abs(l2_g - l2_o) <= 1e-9 * l2_o, (l2_g, l2_o)
+
+
 def test_periodic_heat_512_decays_at_the_analytic_rate():
     """The diffusion configuration at full size (512^2, jitter 0.1 dx, kappa
     0.01, RK4, dt = 0.1 dx^2 / kappa, 200 steps, FP64) against closed forms:
