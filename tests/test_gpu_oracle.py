@@ -75,6 +75,27 @@ def test_taylor_green_64_full_run_matches_oracle():
     assert abs(l2_g - l2_o) <= 1e-9 * l2_o, (l2_g, l2_o)
 
 
+def test_boussinesq_1024_mixed_close_to_fp64():
+    """The convection configuration at full size (1024^2, jitter 0.2 dx,
+    Ra 1e6, WALL_MODEL 3, RK4 + PST): after 20 steps the mixed path's
+    fields are within 1e-6 (norm-wise relative) of the FP64 path's, and no
+    particle needed the FP64 re-solve."""
+    out, stats = [], []
+    for prec in (0, 1):
+        param, st = workloads.boussinesq(n=1024, ra=1e6, jitter=0.2, seed=2030, precision=prec)
+        print(f"[gpu] Boussinesq 1024^2 precision={prec}", flush=True)
+        with simulation_from_state(param, st, precision=prec) as s:
+            s.ghost()
+            s.step(20)
+            out.append(s.download())
+            stats.append(s.stats())
+    n = st.num_int
+    e = _cmp(out[1], out[0], ("x", "y", "u", "v", "rho", "pre", "tem"), n)
+    print(f"[gpu] mixed vs fp64: {e}", flush=True)
+    assert max(e.values()) <= 1e-6, e
+    assert stats[1].fp64_fallbacks == 0 and stats[1].list_overflows == 0
+
+
 def test_periodic_heat_512_decays_at_the_analytic_rate():
     """The diffusion configuration at full size (512^2, jitter 0.1 dx, kappa
     0.01, RK4, dt = 0.1 dx^2 / kappa, 200 steps, FP64) against closed forms:
