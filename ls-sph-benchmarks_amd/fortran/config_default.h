@@ -1,8 +1,8 @@
-/* Default configuration of the GPU driver: the 64 x 64 Taylor-Green vortex
+/* Default configuration of the Fortran program: the 64 x 64 Taylor-Green vortex
    (a = b = 2, Re = 100) in the closed box with free-slip ghost walls, RK2 with
    particle shifting, dt = 1e-3, 100 steps.  Macro names and meanings follow
    the reference's config.h; PERIODIC, PRECISION, DIFF_COEF, DIFF_SOURCE, DT,
-   COND_MAX, JITTER_SEED, HEAT_MODE and USE_GRAPH are this driver's additions
+   COND_MAX, JITTER_SEED, HEAT_MODE and USE_GRAPH are this program's additions
    (defaults in lssph_defaults.h). */
 #define TARGET_PROBLEM 2
 #define U_BOUNDARY_TOP    1
