@@ -68,6 +68,21 @@ def test_create_rejects_walls_in_a_periodic_box():
     assert b"periodic" in lib.lssph_last_error(None)
 
 
+def test_create_rejects_a_periodic_box_under_three_cells():
+    """9 x 9 periodic Taylor-Green: L / h_eff = 2.9, so the 3x3 stencil would
+    visit a column twice; 10 x 10 (3.2) is the smallest accepted box."""
+    from lssph import workloads
+    lib = engine.load_library()
+    param, st = workloads.taylor_green_periodic(n=9, re=100.0, rk=4)
+    p = param.to_c()
+    ni = (ctypes.c_int32 * 1)(st.num_int)
+    ne = (ctypes.c_int32 * 1)(0)
+    h = ctypes.c_void_p()
+    assert lib.lssph_create(ctypes.byref(p), 1, ni, ne, ctypes.byref(h)) == -1
+    assert b"at least 3 cells" in lib.lssph_last_error(None)
+    assert not h.value
+
+
 def test_mismatched_ensemble_options_rejected():
     lib = engine.load_library()
     a, b = _params(rk=4), _params(rk=2)
