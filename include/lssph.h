@@ -199,7 +199,9 @@ int32_t lssph_synchronize(lssph_handle_t h);
    included; a longer list walks the candidates instead; lssph_create sizes
    it from pi (h_eff/dx)^2), and how many time steps (lssph_step, or
    lssph_rk2/lssph_rk4 calls) pass between physical re-sorts of the particle
-   records (default 50; negative: never).  0 leaves a value unchanged. */
+   records (default 50; negative: never).  0 leaves a value unchanged.  The
+   resulting pair must keep (list_len + 1) * block * 4 bytes of LDS within
+   64 KiB; a rejected call changes nothing. */
 int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len,
                          int32_t reorder_interval);
 

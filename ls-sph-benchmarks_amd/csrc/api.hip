@@ -650,6 +650,7 @@ int32_t lssph_download(lssph_handle_t h, int32_t *ptype, double *x, double *y, d
 
 int32_t lssph_download_cell_index(lssph_handle_t h, int32_t *idx_sp) {
   if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!idx_sp) return fail(h, LSSPH_E_ARG, "lssph_download_cell_index: null array");
   if (!h->uploaded) return fail(h, LSSPH_E_STATE, "call lssph_upload first");
   Device &d = h->d;
   launch_download(h->stream, d, h->cur);
@@ -881,17 +882,15 @@ int32_t lssph_synchronize(lssph_handle_t h) {
 
 int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len, int32_t reorder_interval) {
   if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
-  if (block) {
-    if (block < 64 || block > 256 || block % 64)
-      return fail(h, LSSPH_E_ARG, "block must be 64, 128, 192 or 256");
-    h->d.block = block;
-  }
-  if (list_len) {
-    if (list_len < 8 || list_len > 128) return fail(h, LSSPH_E_ARG, "list_len must be in [8, 128]");
-    if ((size_t)(list_len + 1) * (block ? block : h->d.block) * 4 > 64 * 1024)
-      return fail(h, LSSPH_E_ARG, "list_len * block exceeds 64 KiB of LDS");
-    h->d.list_len = list_len;
-  }
+  // every check before any change: a rejected call leaves the handle as it was
+  if (block && (block < 64 || block > 256 || block % 64))
+    return fail(h, LSSPH_E_ARG, "block must be 64, 128, 192 or 256");
+  if (list_len && (list_len < 8 || list_len > 128)) return fail(h, LSSPH_E_ARG, "list_len must be in [8, 128]");
+  const int nb = block ? block : h->d.block, nl = list_len ? list_len : h->d.list_len;
+  if ((size_t)(nl + 1) * nb * 4 > 64 * 1024)
+    return fail(h, LSSPH_E_ARG, "list_len * block exceeds 64 KiB of LDS (the list length kept when list_len is 0)");
+  h->d.block = nb;
+  h->d.list_len = nl;
   if (reorder_interval) h->reorder_interval = reorder_interval < 0 ? 0 : reorder_interval;
   drop_graphs(h);
   return 0;

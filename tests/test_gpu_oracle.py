@@ -218,6 +218,34 @@ def test_list_overflow_path_matches():
             assert rel_err(db[k], da[k]) <= TOL, k
 
 
+def test_tuning_rejects_a_block_the_kept_list_does_not_fit():
+    """The quintic spline (h_eff = 4.65 dx) gets an 86-entry list; block 256
+    with that list needs 87 * 256 * 4 B > 64 KiB, so the call fails and the
+    handle keeps its settings (bitwise the same steps as an untouched one);
+    block 256 with a 63-entry list fits.  A null output array is rejected."""
+    import ctypes
+    from lssph.engine import LssphError
+    param, st = workloads.taylor_green_periodic(n=32, re=100.0, rk=2, jitter=0.2, seed=12, kernel_type=2)
+    with simulation_from_state(param, st) as a, simulation_from_state(param, st) as b:
+        with pytest.raises(LssphError, match="64 KiB"):
+            b.set_tuning(block=256)
+        with pytest.raises(LssphError, match="64 KiB"):
+            b.set_tuning(block=256, list_len=64)
+        a.step(2)
+        b.step(2)
+        da, db = a.download(), b.download()
+        for k in ("x", "y", "u", "v", "rho", "pre"):
+            assert np.array_equal(da[k], db[k]), k
+        b.set_tuning(block=256, list_len=63)
+        a.step(1)
+        b.step(1)
+        da, db = a.download(), b.download()
+        for k in ("x", "y", "u", "v", "rho", "pre"):
+            assert rel_err(db[k], da[k]) <= TOL, k
+        assert b.lib.lssph_download_cell_index(b.h, None) != 0
+        assert b"null" in b.lib.lssph_last_error(b.h)
+
+
 def test_mixed_precision_close_to_fp64():
     param, st = workloads.taylor_green_periodic(n=64, re=1000.0, rk=4, jitter=0.2, seed=13)
     with simulation_from_state(param, st, precision=0) as f64, simulation_from_state(param, st, precision=1) as mix:
