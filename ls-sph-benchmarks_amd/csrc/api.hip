@@ -115,6 +115,10 @@ int validate(const lssph_params_t *p, int n_inst, const int32_t *n_int, const in
     if (!(b.h > 0.0) || !(b.h_eff > 0.0) || !(b.dx > 0.0) || !(b.sp_mass > 0.0) || !(b.len_x > 0.0) ||
         !(b.len_y > 0.0))
       return msg = "h, h_eff, dx, sp_mass, len_x, len_y must be positive", 1;
+    if (!(b.wl_thick >= 0.0)) return msg = "wl_thick must be >= 0", 1;
+    // cells per direction are ints (cell width >= h_eff)
+    if (!((b.len_x + 2.0 * b.wl_thick) / b.h_eff < 1e9) || !((b.len_y + 2.0 * b.wl_thick) / b.h_eff < 1e9))
+      return msg = "problem too large for 32-bit indices: over 1e9 cells per direction", 1;
     if (b.periodic && n_ext[k] != 0) return msg = "a periodic instance has no wall particles", 1;
     if (b.periodic && (std::floor(b.len_x / b.h_eff) < 3 || std::floor(b.len_y / b.h_eff) < 3))
       return msg = "a periodic box needs at least 3 cells of width >= h_eff per direction", 1;
@@ -900,9 +904,14 @@ int32_t lssph_set_skin(lssph_handle_t h, double skin) {
   if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
   if (!(skin >= 0.0) || skin > 1.0) return fail(h, LSSPH_E_ARG, "lssph_set_skin: skin must be in [0, 1]");
   if (h->uploaded) return fail(h, LSSPH_E_STATE, "lssph_set_skin: call before lssph_upload");
+  const double old = h->skin;
   h->skin = skin;
+  if (setup_grid(h) > (1ll << 30)) {  // narrower cells: the grid lssph_create checked can grow
+    h->skin = old;
+    setup_grid(h);
+    return fail(h, LSSPH_E_ARG, "lssph_set_skin: problem too large for 32-bit indices at this skin");
+  }
   drop_graphs(h);
-  setup_grid(h);
   if (int rc = alloc_grid(h)) return rc;
   HIP_TRY(h, hipGetLastError());
   return 0;

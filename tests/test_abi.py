@@ -45,7 +45,8 @@ def _params(**kw):
 
 
 @pytest.mark.parametrize("field,value", [("target_problem", 7), ("kernel_type", 0), ("sph_model", 7),
-                                         ("wall_model", 4), ("rk", 3), ("precision", 2), ("h", -1.0)])
+                                         ("wall_model", 4), ("rk", 3), ("precision", 2), ("h", -1.0),
+                                         ("wl_thick", -1.0), ("h_eff", 1e-12)])
 def test_create_rejects_bad_options_without_touching_the_gpu(field, value):
     lib = engine.load_library()
     p = _params(**{field: value})
