@@ -825,8 +825,12 @@ int32_t lssph_neighbors(lssph_handle_t h, int32_t *counts, int32_t *list, int64_
   int32_t *dlist = nullptr;
   HIP_TRY(h, hipMalloc(&dcounts, sizeof(int32_t) * std::max(d.n_int, 1)));
   launch_neighbor_count(s, d, h->cur, dcounts);
-  HIP_TRY(h, hipMemcpyAsync(counts, dcounts, sizeof(int32_t) * d.n_int, hipMemcpyDeviceToHost, s));
-  HIP_TRY(h, hipStreamSynchronize(s));
+  hipError_t e = hipMemcpyAsync(counts, dcounts, sizeof(int32_t) * d.n_int, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(dcounts);
+    return fail(h, LSSPH_E_HIP, std::string("lssph_neighbors: ") + hipGetErrorString(e));
+  }
   std::vector<int64_t> off(d.n_int + 1, 0);
   for (int i = 0; i < d.n_int; ++i) off[i + 1] = off[i] + counts[i];
   const int64_t tot = off[d.n_int];

@@ -97,6 +97,9 @@ def load_library(path: Path | None = None):
     return lib
 
 
+LSSPH_E_CAPACITY = -6  # lssph.h: caller buffer too small
+
+
 def _dp(a):
     return None if a is None else a.ctypes.data_as(_D)
 
@@ -125,6 +128,9 @@ class Simulation:
         self.params = cparams
         self.num_int = [int(n) for n in num_int]
         self.num_ext = [int(n) for n in num_ext]
+        if len(self.num_int) != self.n_inst or len(self.num_ext) != self.n_inst:
+            raise ValueError(f"{self.n_inst} parameter sets but {len(self.num_int)} fluid and "
+                             f"{len(self.num_ext)} wall counts")
         self.n_int = sum(self.num_int)
         self.n_ext = sum(self.num_ext)
         self.n_total = self.n_int + self.n_ext
@@ -169,8 +175,9 @@ class Simulation:
         tem = get("tem") if self.has_tem else None
         tem = None if tem is None else np.ascontiguousarray(tem, dtype=np.float64)
         pt = np.ascontiguousarray(get("ptype"), dtype=np.int32)
-        if pt.size != self.n_total:
-            raise ValueError(f"state has {pt.size} particles, handle expects {self.n_total}")
+        for name, a in zip(("ptype", "x", "y", "u", "v", "rho", "pre", "tem"), [pt] + arrs + [tem]):
+            if a is not None and a.size != self.n_total:
+                raise ValueError(f"state {name} has {a.size} entries, handle expects {self.n_total}")
         self._check(self.lib.lssph_upload(self.h, _ip(pt), *(_dp(a) for a in arrs), _dp(tem)))
 
     def download(self) -> dict:
@@ -249,9 +256,10 @@ class Simulation:
         counts = np.zeros(self.n_int, np.int32)
         total = ctypes.c_int64()
         rc = self.lib.lssph_neighbors(self.h, _ip(counts), None, 0, ctypes.byref(total))
+        if rc != LSSPH_E_CAPACITY:  # the sizing call (no list) expects only "capacity too small"
+            self._check(rc)
         lst = np.zeros(max(total.value, 1), np.int32)
         self._check(self.lib.lssph_neighbors(self.h, _ip(counts), _ip(lst), total.value, ctypes.byref(total)))
-        del rc
         return counts, lst[: total.value]
 
     def stats(self) -> Stats:

@@ -103,3 +103,10 @@ def test_params_struct_layout_matches_header():
 def test_set_scan_rejects_a_null_handle():
     lib = engine.load_library()
     assert lib.lssph_set_scan(None, 1) == -1
+
+
+def test_simulation_rejects_counts_that_do_not_match_the_instances():
+    from lssph import workloads
+    param, st = workloads.taylor_green_periodic(n=16, re=100.0, rk=2)
+    with pytest.raises(ValueError, match="parameter sets"):
+        engine.Simulation(param, [st.num_int, st.num_int], [0])

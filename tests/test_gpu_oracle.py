@@ -246,6 +246,19 @@ def test_tuning_rejects_a_block_the_kept_list_does_not_fit():
         assert b"null" in b.lib.lssph_last_error(b.h)
 
 
+def test_upload_checks_every_field_size():
+    """Every array of an upload must hold n_total entries; a short one is
+    refused before the library reads it, and the handle stays usable."""
+    param, st = workloads.taylor_green_periodic(n=16, re=100.0, rk=2, jitter=0.2, seed=3)
+    with simulation_from_state(param, st) as s:
+        bad = {k: getattr(st, k) for k in ("ptype", "x", "y", "u", "v", "rho", "pre")}
+        bad["rho"] = bad["rho"][:-1]
+        with pytest.raises(ValueError, match="rho"):
+            s.upload(bad)
+        counts, lst = s.neighbors()
+        assert counts.size == st.num_int and lst.size == counts.sum() > 0
+
+
 def test_mixed_precision_close_to_fp64():
     param, st = workloads.taylor_green_periodic(n=64, re=1000.0, rk=4, jitter=0.2, seed=13)
     with simulation_from_state(param, st, precision=0) as f64, simulation_from_state(param, st, precision=1) as mix:
