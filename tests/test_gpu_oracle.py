@@ -246,6 +246,45 @@ def test_tuning_rejects_a_block_the_kept_list_does_not_fit():
         assert b"null" in b.lib.lssph_last_error(b.h)
 
 
+def test_misuse_returns_error_codes_and_leaves_the_handle_working():
+    """Calls out of order return LSSPH_E_STATE (-3) and bad values
+    LSSPH_E_ARG (-1), each with a message; none changes the handle, which
+    then steps bitwise like one that saw no bad call."""
+    from lssph.engine import LssphError
+    fields = ("ptype", "x", "y", "u", "v", "rho", "pre")
+    param, st = workloads.taylor_green_periodic(n=24, re=100.0, rk=4, jitter=0.2, seed=21)
+
+    def code(f, *a):
+        with pytest.raises(LssphError) as e:
+            f(*a)
+        assert str(e.value).split(": ", 1)[1]
+        return e.value.code
+
+    with Simulation(param, [st.num_int], [st.num_ext]) as s, simulation_from_state(param, st) as ref:
+        assert code(s.step, 1) == -3  # before lssph_upload
+        assert code(s.calculate_RHS) == -3
+        assert code(s.download) == -3
+        s.upload(st)
+        assert code(s.step, 1) == -3  # before lssph_init_cell
+        s.init_cell()
+        assert code(s.RK2) == -1  # the handle was created for RK4
+        assert code(s.step, -1) == -1
+        assert code(s.set_skin, 0.1) == -3  # the cell grid is fixed at upload
+        assert s.lib.lssph_set_scan(s.h, 2) == -1
+        assert code(s.set_tuning, 32) == -1
+        bad = {k: getattr(st, k).copy() for k in fields}
+        bad["rho"][3] = 0.0
+        assert code(s.upload, bad) == -1
+        bad = {k: getattr(st, k).copy() for k in fields}
+        bad["ptype"][0] = 1  # a wall type in the fluid block
+        assert code(s.upload, bad) == -1
+        s.step(3)
+        ref.step(3)
+        a, b = s.download(), ref.download()
+        for k in fields[1:]:
+            assert np.array_equal(a[k], b[k]), k
+
+
 def test_upload_checks_every_field_size():
     """Every array of an upload must hold n_total entries; a short one is
     refused before the library reads it, and the handle stays usable."""
