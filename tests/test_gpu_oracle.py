@@ -312,6 +312,34 @@ def test_mixed_precision_close_to_fp64():
             assert rel_err(b[k], a[k]) < 1e-4, k
 
 
+@pytest.mark.parametrize("box", ["periodic", "walls"])
+@pytest.mark.parametrize("kernel_type", [1, 2, 4, 5])
+def test_mixed_precision_close_to_fp64_other_kernels(kernel_type, box):
+    """The mixed path's single-precision kernel shapes (cubic spline,
+    quintic spline, Wendland C4, C6; Wendland C2 is covered above) against
+    the FP64 path, which the reference's fixtures pin for each.  After 10
+    steps the measured norm-wise differences are at most 4.7e-8 (u, v) and
+    1.2e-8 (rho) over the eight cases; the bounds leave about 20x room, while
+    a wrong shape, derivative or normalisation constant is off by far more."""
+    if box == "periodic":
+        param, st = workloads.taylor_green_periodic(n=64, re=1000.0, rk=4, jitter=0.2, seed=19,
+                                                    kernel_type=kernel_type)
+    else:
+        param, st = workloads.taylor_green_box(n=64, re=100.0, rk=2, jitter=0.2, seed=19, kernel_type=kernel_type)
+    print(f"[gpu] mixed vs FP64, kernel_type={kernel_type} {box}", flush=True)
+    out = []
+    for prec in (0, 1):
+        with simulation_from_state(param, st, precision=prec) as s:
+            if box == "walls":
+                s.ghost()
+            s.step(10)
+            out.append(s.download())
+    n = st.num_int
+    e = {k: rel_err(out[1][k][:n], out[0][k][:n]) for k in ("u", "v", "rho")}
+    print(f"[gpu] mixed vs FP64: {e}", flush=True)
+    assert e["u"] < 1e-6 and e["v"] < 1e-6 and e["rho"] < 3e-7, e
+
+
 def test_mixed_fp64_fallback_path_matches_fp64():
     """A condition limit below every pivot ratio sends every mixed solve to
     its FP64 fallback: k_lsa's list-driven FP64 relaunch and the shifting's
