@@ -110,3 +110,28 @@ def test_diffusion_steady_flags():
             s.eager_step(1)
             flags.append(int(s.check_steady_state()[0][0]))
         assert flags == list(g["steady"])
+
+
+def _ls_q5_cases():
+    return [c for c in CASES if Golden(c).cfg.sph_model == 3]
+
+
+@pytest.mark.parametrize("case", _ls_q5_cases())
+def test_mixed_path_tracks_the_reference_trajectory(case):
+    """The mixed path (SPH_MODEL 3 only) from the reference's own input
+    state, over the reference's run (6 or 10 steps), against its final
+    state.  Measured maxima over the nine cases: x, y 5.8e-9; u, v 1.9e-7;
+    rho 1.8e-8; pre 2.6e-6 (the EOS amplifies the density's error); tem
+    1.1e-10.  The bounds leave about 20x room.  No particle takes the FP64
+    re-solve, so the single-precision sums are what is measured."""
+    g = Golden(case)
+    with _sim(g, precision=1) as s:
+        s.ghost()
+        s.step(g.steps)
+        e = max_rel(s.download(), g, "step_last")
+        fb = s.stats().fp64_fallbacks
+    print(f"[gpu] mixed vs reference, {case}: {e} fallbacks={fb}", flush=True)
+    bound = {"x": 1e-7, "y": 1e-7, "pre": 5e-5}
+    for k, v in e.items():
+        assert v <= bound.get(k, 3e-6), (k, e)
+    assert fb == 0
