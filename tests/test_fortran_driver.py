@@ -32,7 +32,10 @@ DRIVER_CASES = {
     "bous_rk4_wm3_restart": "bous_rk4_wm3",
     "tg_sph2_rk4_delta_restart": "tg_sph2_rk4_delta",
     "tg_ls9_rk2_restart": "tg_ls9_rk2",
+    "tg_rk4_delta_mixed_restart": "tg_rk4_delta",
 }
+# driver configurations that also choose the arithmetic (an extension macro)
+ARITHMETIC = {"tg_rk4_delta_mixed_restart": {"PRECISION"}}
 # macros that only set the run's schedule, names and launch mode
 SCHEDULE = {"SAVE_NAME", "READ_NAME", "START_STEP", "END_STEP", "WRITE_STEP", "USE_GRAPH"}
 
@@ -54,7 +57,7 @@ def save_name(case: str) -> str:
 def test_driver_configs_are_the_fixture_configs(case):
     g = Golden(DRIVER_CASES[case])
     h = header_macros(case)
-    differ = {k for k in set(h) | set(g.macros) if h.get(k) != g.macros.get(k)} - SCHEDULE
+    differ = {k for k in set(h) | set(g.macros) if h.get(k) != g.macros.get(k)} - SCHEDULE - ARITHMETIC.get(case, set())
     assert not differ, f"{case}.h differs from the fixture's configuration in {sorted(differ)}"
 
 
@@ -85,10 +88,11 @@ def test_driver_setup_matches_reference(case, tmp_path):
         assert par[key] == g.ref_params[key], key
 
 
-def _compare(got: dict, ref: Golden, tag: str, label: str):
+def _compare(got: dict, ref: Golden, tag: str, label: str, tol=None):
     for k in ref.fields():
         err = rel_err(got[k], ref[f"{tag}_{k}"])
-        assert err <= TOL, f"{label}: {k} differs from the reference's {tag} by {err:.3e}"
+        bound = TOL if tol is None else tol.get(k, tol["*"])
+        assert err <= bound, f"{label}: {k} differs from the reference's {tag} by {err:.3e}"
     np.testing.assert_array_equal(got["ptype"], ref[f"{tag}_ptype"])
 
 
@@ -138,6 +142,26 @@ def test_driver_restart(case, tmp_path):
     assert not (data / "0.dat").exists()
     for k, tag in ((1, "step_1"), (2, "step_2"), (g.steps, "step_last")):
         _compare(read_state(data / f"{1 + k}.dat", g.n_total, g.has_tem), g, tag, f"{1 + k}.dat")
+
+
+@pytest.mark.gpu
+def test_driver_restart_mixed_precision(tmp_path):
+    """The same restart with PRECISION 1 in config.h: the driver runs the
+    mixed path, and its states stay within the mixed path's bounds of the
+    reference's FP64 states (the library's own fixture test measures at most
+    1.9e-7 on u, v and 2.6e-6 on p for these cases)."""
+    case = "tg_rk4_delta_mixed_restart"
+    g = Golden(DRIVER_CASES[case])
+    write_state(tmp_path / "results" / "inject" / "data" / "1.dat", g.state("state0"), g.has_tem)
+    r = run(case, tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    data = tmp_path / "results" / save_name(case) / "data"
+    tol = {"x": 1e-7, "y": 1e-7, "pre": 5e-5, "*": 3e-6}
+    for k, tag in ((1, "step_1"), (2, "step_2"), (g.steps, "step_last")):
+        _compare(read_state(data / f"{1 + k}.dat", g.n_total, g.has_tem), g, tag, f"{1 + k}.dat", tol)
+    # the mixed path ran: FP64 would match the reference to 1e-12
+    last = read_state(data / f"{1 + g.steps}.dat", g.n_total, g.has_tem)
+    assert max(rel_err(last[k], g[f"step_last_{k}"]) for k in ("u", "v")) > 1e-12
 
 
 @pytest.mark.gpu
