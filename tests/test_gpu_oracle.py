@@ -388,6 +388,34 @@ def test_cavity_ensemble_matches_oracle_per_instance():
         assert max(e.values()) <= TOL, e
 
 
+def test_boussinesq_ensemble_matches_oracle_per_instance():
+    """Three convection boxes of different sizes and Rayleigh numbers in one
+    handle (temperature field, adiabatic and fixed-temperature walls,
+    WALL_MODEL 3): each instance matches the oracle run alone."""
+    cases = [(16, 1e4, 31), (24, 1e5, 32), (20, 1e6, 33)]
+    params, states = [], []
+    for n, ra, seed in cases:
+        p, st = workloads.boussinesq(n=n, ra=ra, jitter=0.2, seed=seed, precision=0)
+        params.append(p)
+        states.append(st)
+    print(f"[gpu] Boussinesq ensemble {cases}", flush=True)
+    ni = [s.num_int for s in states]
+    ne = [s.num_ext for s in states]
+    with Simulation(params, ni, ne, precision=0) as sim:
+        sim.upload(states)
+        sim.init_cell()
+        sim.ghost()
+        sim.step(3)
+        out = split_states(sim.download(), ni, ne)
+    for p, st, got in zip(params, states, out):
+        o = OracleSim(p.to_c(), st.num_int, st.num_ext)
+        o.init(st)
+        o.ghost()
+        o.step(3)
+        e = _cmp(got, o.get_state(), ("x", "y", "u", "v", "rho", "pre", "tem"))
+        assert max(e.values()) <= TOL, e
+
+
 def test_mixed_ensemble_equals_separate_handles_bitwise():
     """On the mixed path an instance's arithmetic does not depend on the
     others in its handle (waves spanning two instances take the per-lane
