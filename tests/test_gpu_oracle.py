@@ -416,13 +416,20 @@ def test_boussinesq_ensemble_matches_oracle_per_instance():
         assert max(e.values()) <= TOL, e
 
 
-def test_mixed_ensemble_equals_separate_handles_bitwise():
+@pytest.mark.parametrize("kind", ["cavity", "boussinesq"])
+def test_mixed_ensemble_equals_separate_handles_bitwise(kind):
     """On the mixed path an instance's arithmetic does not depend on the
     others in its handle (waves spanning two instances take the per-lane
-    copy with the same operations): five cavities stepped together equal
-    each stepped in a handle of its own, bit for bit."""
-    params, states, meta = workloads.cavity_ensemble(count=5, sizes=(12, 16, 20), seed=6, precision=1)
-    print(f"[gpu] mixed ensemble sizes={[m['n'] for m in meta]}", flush=True)
+    copy with the same operations): five cavities, or three convection
+    boxes with their temperature field, stepped together equal each stepped
+    in a handle of its own, bit for bit."""
+    if kind == "cavity":
+        params, states, meta = workloads.cavity_ensemble(count=5, sizes=(12, 16, 20), seed=6, precision=1)
+    else:
+        pairs = [workloads.boussinesq(n=n, ra=ra, jitter=0.2, seed=sd, precision=1)
+                 for n, ra, sd in ((16, 1e4, 41), (24, 1e5, 42), (20, 1e6, 43))]
+        params, states = [p for p, _ in pairs], [st for _, st in pairs]
+    print(f"[gpu] mixed {kind} ensemble n_int={[st.num_int for st in states]}", flush=True)
     ni = [s.num_int for s in states]
     ne = [s.num_ext for s in states]
     with Simulation(params, ni, ne, precision=1) as sim:
@@ -436,7 +443,7 @@ def test_mixed_ensemble_equals_separate_handles_bitwise():
             one.ghost()
             one.step(4)
             ref = one.download()
-        for k in ("x", "y", "u", "v", "rho", "pre"):
+        for k in ("x", "y", "u", "v", "rho", "pre") + (("tem",) if kind == "boussinesq" else ()):
             assert np.array_equal(got[k], ref[k]), k
 
 
