@@ -447,6 +447,32 @@ def test_mixed_ensemble_equals_separate_handles_bitwise(kind):
             assert np.array_equal(got[k], ref[k]), k
 
 
+def test_heat_ensemble_steady_check_per_instance():
+    """Diffusion boxes of different sizes and diffusivities in one handle:
+    fields and the per-instance steady-state maxima max|f - f_prev| equal
+    those of each box in a handle of its own, bit for bit."""
+    pairs = [workloads.heat_periodic(n=n, kappa=k, jitter=0.1, seed=sd)
+             for n, k, sd in ((24, 0.01, 51), (32, 0.02, 52), (40, 0.005, 53))]
+    params, states = [p for p, _ in pairs], [st for _, st in pairs]
+    print(f"[gpu] heat ensemble n_int={[st.num_int for st in states]}", flush=True)
+    ni = [s.num_int for s in states]
+    ne = [s.num_ext for s in states]
+    with Simulation(params, ni, ne, precision=0) as sim:
+        sim.upload(states)
+        sim.init_cell()
+        sim.step(3)
+        flags, df = sim.check_steady_state()
+        out = split_states(sim.download(), ni, ne)
+    assert flags.shape == (3,) and np.all(df > 0.0)
+    for k, (p, st) in enumerate(pairs):
+        with simulation_from_state(p, st, precision=0) as one:
+            one.step(3)
+            f1, d1 = one.check_steady_state()
+            ref = one.download()
+        assert flags[k] == f1[0] and df[k] == d1[0], (k, df[k], d1[0])
+        assert np.array_equal(out[k]["u"], ref["u"]), k
+
+
 def test_large_ensemble_ghost_path_matches_oracle():
     """22 cavities of 256^2 carry 68 K virtual markers, which the ghost
     launcher serves with 2 lanes per marker (4 below 64 K, as in the smaller
