@@ -182,7 +182,9 @@ int32_t lssph_download_cell_index(lssph_handle_t h, int32_t *idx_sp);
    r_ij^2 <= h_eff^2 } of every fluid particle (LSSPH_A_mod.f90:102-124), in
    summation order, as 0-based upload-order indices.  counts[sum(n_int)];
    list receives the concatenation (capacity entries available); *total gets
-   the number written.  Synchronises. */
+   the number written.  With list NULL or capacity below the total, counts
+   and *total are still filled and the call returns LSSPH_E_CAPACITY (the
+   sizing call).  Synchronises. */
 int32_t lssph_neighbors(lssph_handle_t h, int32_t *counts, int32_t *list,
                         int64_t capacity, int64_t *total);
 
