@@ -67,9 +67,11 @@ typedef struct lssph_params {
   /* extensions (reference value in brackets) */
   double diff_coef;         /* factor on the Laplacian in the diffusion RHS [1] */
   double diff_source;       /* factor on the manufactured source term       [1] */
-  double cond_max;          /* mixed precision: pivot-ratio condition estimate
-                               above which a particle is re-solved in full
-                               FP64 (<= 0 selects 1e4)                        */
+  double cond_max;          /* mixed precision: a particle whose 5x5 moment
+                               matrix has a 1-norm condition number
+                               ||M||_1 ||M^-1||_1 (exact, from the LDL^T
+                               factors) above this is redone in full FP64,
+                               sums included (<= 0 selects 1e4)             */
   /* model options (config.h macros) */
   int32_t target_problem;   /* TARGET_PROBLEM 1..4                            */
   int32_t kernel_type;      /* KERNEL_TYPE 1..5                               */
@@ -89,7 +91,9 @@ typedef struct lssph_params {
 
 /* Counters the device keeps; read with lssph_stats(). */
 typedef struct lssph_stats {
-  int64_t fp64_fallbacks;   /* mixed path: particles re-solved in FP64        */
+  int64_t fp64_fallbacks;   /* mixed path: particle passes (operator or
+                               shift) redone in full FP64 after the
+                               condition check                              */
   int64_t list_overflows;   /* particles whose neighbor list exceeded the
                                on-chip list and took the direct path          */
   int32_t max_neighbors;    /* largest neighbor count seen                    */
@@ -187,6 +191,61 @@ int32_t lssph_download_cell_index(lssph_handle_t h, int32_t *idx_sp);
    sizing call).  Synchronises. */
 int32_t lssph_neighbors(lssph_handle_t h, int32_t *counts, int32_t *list,
                         int64_t capacity, int64_t *total);
+
+/* Validation of the pair kernels' own neighbor lists (no reference
+   counterpart; the sets are those of LSSPH_A_mod.f90:102-124).  Runs the
+   candidate scan and the list walk of the LS operator and shifting kernels
+   (same code, launch shape and on-chip layout, under the current
+   lssph_set_scan / lssph_set_tuning settings) and records, per fluid particle
+   in upload order, the neighbors the walk visits, in its order:
+   list[i * cap + k], k < counts[i], holds 0-based upload indices (-1 - j
+   would mark a visit beyond h_eff); path[i] is the scan path taken: 0
+   per-particle scan, 1 wave-staged scan row by row, 2 wave-staged scan with
+   positions kept on chip, 3 list overflow (direct stencil walk).  Returns
+   LSSPH_E_CAPACITY when some counts[i] > cap (counts are still filled).
+   Not for SPH_MODEL 6, whose kernels gather per wavefront.  Synchronises. */
+int32_t lssph_pair_lists(lssph_handle_t h, int32_t cap, int32_t *counts,
+                         int32_t *list, int32_t *path);
+
+/* Type-B least-squares fit of one field at n probe points of instance inst,
+   as the reference's analysis interpolates wall values and gradients
+   (analysis/common/lssph_b.py:21-144, used for the Nusselt number by
+   analysis/benchmarks/boussinesq_conv.py:80-108): all particles, fluid and
+   wall, within h_eff of a probe; q = 3, 6 or 10 basis terms with a constant.
+   x, y are in the upload coordinates (walls from 0 to wl_thick); field 0 u,
+   1 v, 2 rho, 3 pre, 4 tem.  out[3k], out[3k+1], out[3k+2] = value, d/dx,
+   d/dy at probe k (NaN with fewer than q particles or a singular fit).
+   FP64 in either precision mode.  Synchronises. */
+int32_t lssph_probe(lssph_handle_t h, int32_t inst, int32_t field, int32_t q,
+                    int32_t n, const double *x, const double *y, double *out);
+
+/* Ensembles: isolation of diverged instances (no reference counterpart: the
+   reference runs one box and, when its scheme diverges, indexes past its cell
+   arrays).  on = 1: a fluid particle that leaves its instance's cell grid,
+   or moves farther than the cell-list skin allows between builds (a
+   non-finite position does both), retires its instance instead of failing
+   the handle.  From the next pass on, the pair kernels and ghost walls leave
+   that instance's records as they are; the other instances run on, and
+   their results are the same as without the retired one.  0 (the default):
+   such a move is an LSSPH_E_DOMAIN error of the handle.  Turning it on
+   clears every instance's status and restarts the step count; turning it
+   off leaves instances already retired frozen (lssph_upload revives them
+   with new data).  A too small skin (lssph_set_skin) for an instance's
+   velocities also retires it: its moves outrun the cell list. */
+int32_t lssph_set_isolation(lssph_handle_t h, int32_t on);
+
+/* retired[n_inst]: 0 while instance k runs, else the time step (counted
+   from lssph_set_isolation, one per lssph_rk2/lssph_rk4 call or lssph_step
+   iteration) during which it was retired; reason[n_inst] (may be NULL): the
+   first cause seen, LSSPH_RETIRED_SKIN (a move beyond the cell-list skin:
+   the instance's neighbor sets could no longer be guaranteed -- a diverging
+   run, or a skin too small for the instance's velocities) or
+   LSSPH_RETIRED_ESCAPED (a position outside the grid or not finite).
+   Synchronises. */
+#define LSSPH_RETIRED_SKIN 1
+#define LSSPH_RETIRED_ESCAPED 2
+int32_t lssph_instance_status(lssph_handle_t h, int32_t *retired,
+                              int32_t *reason);
 
 /* Device counters (see lssph_stats_t).  Synchronises. */
 int32_t lssph_stats(lssph_handle_t h, lssph_stats_t *out);

@@ -315,6 +315,8 @@ void enqueue_stage(lssph_handle *h, StageMode m) {
 }
 
 void enqueue_rk(lssph_handle *h) {
+  // isolation mode: the device's step number, which a retirement records
+  if (h->d.retired) launch_add_i32(h->stream, h->d.counters + kStepNo, 1);
   if (h->d.opt.rk == 2) {
     enqueue_stage(h, kRk2S1);
     enqueue_stage(h, kRk2S2);
@@ -464,7 +466,14 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
     ne += n_ext[k];
   }
   const int64_t nc = setup_grid(h);
-  if (ni + ne > (1ll << 30) || nc > (1ll << 30) || n_inst >= (1 << 23)) {
+  if (ni + ne > kMaxParticles) {
+    delete h;
+    return fail(nullptr, LSSPH_E_ARG,
+                "lssph_create: more than " + std::to_string(kMaxParticles) +
+                    " particles in one handle (neighbor-list entries hold 27-bit slots); split the problem "
+                    "over several handles");
+  }
+  if (nc > (1ll << 30) || n_inst >= (1 << 23)) {
     delete h;
     return fail(nullptr, LSSPH_E_ARG, "lssph_create: problem too large for 32-bit indices");
   }
@@ -546,6 +555,7 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   rc = rc ? rc : dalloc(h, &d.dfmax, n_inst);
   rc = rc ? rc : dalloc(h, &d.steady, n_inst);
   rc = rc ? rc : dalloc(h, &d.counters, kNumCounters);
+  rc = rc ? rc : dalloc(h, &d.retired_buf, 2 * n_inst);
   rc = rc ? rc : dalloc(h, &d.stats64, kNumStats64);
   rc = rc ? rc : dalloc(h, &d.stage_f64, 8 * n);
   rc = rc ? rc : dalloc(h, &d.stage_i32, n);
@@ -614,6 +624,8 @@ int32_t lssph_upload(lssph_handle_t h, const int32_t *ptype, const double *x, co
   launch_iota(s, d.items, (int)n);
   launch_iota(s, d.slot_of, (int)n);
   launch_fill_i32(s, d.counters, 0, kNumCounters);
+  launch_fill_i32(s, d.retired_buf, 0, 2 * d.n_inst);
+  HIP_TRY(h, hipMemcpyAsync(d.inst, h->inst_host.data(), sizeof(InstDev) * d.n_inst, hipMemcpyHostToDevice, s));
   launch_fill_i32(s, d.cell_count, 0, d.n_cells);
   h->cur = 0;
   full_build(h, 1);
@@ -857,6 +869,65 @@ int32_t lssph_neighbors(lssph_handle_t h, int32_t *counts, int32_t *list, int64_
   return rc;
 }
 
+int32_t lssph_pair_lists(lssph_handle_t h, int32_t cap, int32_t *counts, int32_t *list, int32_t *path) {
+  if (int rc = ready(h)) return rc;
+  if (!counts || !list || !path || cap < 1) return fail(h, LSSPH_E_ARG, "lssph_pair_lists: null array or cap < 1");
+  if (h->d.opt.sph_model == 6)
+    return fail(h, LSSPH_E_ARG, "lssph_pair_lists: the SPH_MODEL 6 kernels gather per wavefront (lssph_neighbors)");
+  Device &d = h->d;
+  hipStream_t s = h->stream;
+  const size_t ni = std::max(d.n_int, 1);
+  int32_t *dc = nullptr, *dl = nullptr, *dp = nullptr;
+  int rc = 0;
+  if (hipMalloc(&dc, sizeof(int32_t) * ni) != hipSuccess || hipMalloc(&dp, sizeof(int32_t) * ni) != hipSuccess ||
+      hipMalloc(&dl, sizeof(int32_t) * ni * (size_t)cap) != hipSuccess) {
+    rc = fail(h, LSSPH_E_NOMEM, "lssph_pair_lists: hipMalloc failed");
+  } else {
+    (void)hipMemsetAsync(dc, 0, sizeof(int32_t) * ni, s);
+    (void)hipMemsetAsync(dp, 0xff, sizeof(int32_t) * ni, s);
+    launch_pair_lists(s, d, h->cur, dc, dl, dp, cap);
+    hipError_t e = hipMemcpyAsync(counts, dc, sizeof(int32_t) * d.n_int, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(path, dp, sizeof(int32_t) * d.n_int, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(list, dl, sizeof(int32_t) * d.n_int * (size_t)cap, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = fail(h, LSSPH_E_HIP, std::string("lssph_pair_lists: ") + hipGetErrorString(e));
+  }
+  if (dc) (void)hipFree(dc);
+  if (dl) (void)hipFree(dl);
+  if (dp) (void)hipFree(dp);
+  if (rc) return rc;
+  for (int i = 0; i < d.n_int; ++i)
+    if (counts[i] > cap) return fail(h, LSSPH_E_CAPACITY, "lssph_pair_lists: a list is longer than cap");
+  return 0;
+}
+
+int32_t lssph_probe(lssph_handle_t h, int32_t inst, int32_t field, int32_t q, int32_t n, const double *x,
+                    const double *y, double *out) {
+  if (int rc = ready(h)) return rc;
+  Device &d = h->d;
+  if (inst < 0 || inst >= d.n_inst) return fail(h, LSSPH_E_ARG, "lssph_probe: instance out of range");
+  if (field < 0 || field > 4 || (field == 4 && !d.opt.has_tem))
+    return fail(h, LSSPH_E_ARG, "lssph_probe: field must be 0..3, or 4 (tem) for target_problem 4");
+  if (q != 3 && q != 6 && q != 10) return fail(h, LSSPH_E_ARG, "lssph_probe: q must be 3, 6 or 10");
+  if (n < 0 || (n > 0 && (!x || !y || !out))) return fail(h, LSSPH_E_ARG, "lssph_probe: null array or n < 0");
+  if (n == 0) return 0;
+  hipStream_t s = h->stream;
+  double *buf = nullptr;
+  if (hipMalloc(&buf, sizeof(double) * 5 * (size_t)n) != hipSuccess)
+    return fail(h, LSSPH_E_NOMEM, "lssph_probe: hipMalloc failed");
+  hipError_t e = hipMemcpyAsync(buf, x, sizeof(double) * n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(buf + n, y, sizeof(double) * n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    launch_probe(s, d, h->cur, inst, field, q, n, buf, buf + n, buf + 2 * n);
+    e = hipMemcpyAsync(out, buf + 2 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(h, LSSPH_E_HIP, std::string("lssph_probe: ") + hipGetErrorString(e));
+  return check_counters(h);
+}
+
 int32_t lssph_stats(lssph_handle_t h, lssph_stats_t *out) {
   if (!h || !out) return fail(h, LSSPH_E_ARG, "lssph_stats: null argument");
   int32_t c[kNumCounters];
@@ -918,6 +989,37 @@ int32_t lssph_set_skin(lssph_handle_t h, double skin) {
   drop_graphs(h);
   if (int rc = alloc_grid(h)) return rc;
   HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_set_isolation(lssph_handle_t h, int32_t on) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (on != 0 && on != 1) return fail(h, LSSPH_E_ARG, "lssph_set_isolation: on must be 0 or 1");
+  Device &d = h->d;
+  if ((d.retired != nullptr) == (on != 0)) return 0;
+  if (on) {
+    // a fresh start: no instance retired, steps counted from here
+    HIP_TRY(h, hipMemcpyAsync(d.inst, h->inst_host.data(), sizeof(InstDev) * d.n_inst, hipMemcpyHostToDevice,
+                              h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    launch_fill_i32(h->stream, d.retired_buf, 0, 2 * d.n_inst);
+    launch_fill_i32(h->stream, d.counters + kStepNo, 0, 1);
+    HIP_TRY(h, hipGetLastError());
+  }
+  d.retired = on ? d.retired_buf : nullptr;
+  drop_graphs(h);
+  return 0;
+}
+
+int32_t lssph_instance_status(lssph_handle_t h, int32_t *retired, int32_t *reason) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!retired) return fail(h, LSSPH_E_ARG, "lssph_instance_status: null array");
+  Device &d = h->d;
+  HIP_TRY(h, hipMemcpyAsync(retired, d.retired_buf, sizeof(int32_t) * d.n_inst, hipMemcpyDeviceToHost, h->stream));
+  if (reason)
+    HIP_TRY(h, hipMemcpyAsync(reason, d.retired_buf + d.n_inst, sizeof(int32_t) * d.n_inst, hipMemcpyDeviceToHost,
+                              h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 

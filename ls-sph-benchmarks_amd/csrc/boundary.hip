@@ -220,6 +220,7 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs A) {
   if (k >= A.n_ext) return;
   const Options &o = A.opt;
   const InstDev &I = A.inst[A.vm_inst[k]];
+  if (I.retired) return;  // a retired instance: the whole group leaves
   const bool has_t = o.has_tem != 0;
   double val[4];
   if constexpr (WM == 1)
@@ -351,6 +352,110 @@ __global__ void k_vm_init(int n_int, int n_ext, const Rec *__restrict__ rec, con
   vm_wall[k] = pw;
 }
 
+// ---------------------------------------------------------------------------
+// Type-B LS fit of one field at probe points, the interpolation the
+// reference's analysis uses for wall gradients (analysis/common/lssph_b.py:
+// 21-144, called by analysis/benchmarks/boussinesq_conv.py:91-99 for the
+// Nusselt number): every particle -- fluid and wall -- within h_eff of the
+// probe, weights W(r) m / rho_j, basis a = [1, x/h, y/h (, x^2/2h^2, xy/h^2,
+// y^2/2h^2 (, x^3/6h^3, x^2y/2h^3, xy^2/2h^3, y^3/6h^3))], offsets x = x_j -
+// x_probe; out = (d_0, d_1 / h, d_2 / h), NaN when fewer than q particles
+// are found or the moment matrix is singular.  FP64, one thread per probe.
+// ---------------------------------------------------------------------------
+struct ProbeArgs {
+  const Rec *__restrict__ mir;
+  const int32_t *__restrict__ cstart;
+  const InstDev *__restrict__ inst;
+  const double *__restrict__ px, *__restrict__ py;
+  double *out;
+  int n, inst_id, field, kt;
+};
+
+template <int Q>
+__device__ __forceinline__ void probe_basis(double x, double y, double ih, double (&a)[Q]) {
+  const double xh = x * ih, yh = y * ih;
+  a[0] = 1.0;
+  a[1] = xh;
+  a[2] = yh;
+  if constexpr (Q >= 6) {
+    a[3] = 0.5 * xh * xh;
+    a[4] = xh * yh;
+    a[5] = 0.5 * yh * yh;
+  }
+  if constexpr (Q >= 10) {
+    a[6] = (1.0 / 6.0) * xh * xh * xh;
+    a[7] = 0.5 * xh * xh * yh;
+    a[8] = 0.5 * xh * yh * yh;
+    a[9] = (1.0 / 6.0) * yh * yh * yh;
+  }
+}
+
+__device__ __forceinline__ double rec_field(const Rec &r, int f) {
+  switch (f) {
+    case 0: return r.u;
+    case 1: return r.v;
+    case 2: return r.rho;
+    case 3: return r.pre;
+    default: return r.tem;
+  }
+}
+
+template <int Q>
+__global__ void __launch_bounds__(64) k_probe_b(ProbeArgs A) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.n) return;
+  const InstDev &I = A.inst[A.inst_id];
+  const double mx = A.px[k], my = A.py[k];
+  const double nan = __longlong_as_double(0x7ff8000000000000ll);
+  const int c = fixed_cell(I, mx, my);
+  if (c < 0) {
+    A.out[3 * k] = A.out[3 * k + 1] = A.out[3 * k + 2] = nan;
+    return;
+  }
+  double M[Q][Q], b[1][Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    b[0][i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) M[i][j] = 0.0;
+  }
+  int num = 0;
+  const bool per = I.periodic;
+  visit_cells(I, c, A.cstart, [&](int t0, int t1, int, int) {
+    for (int t = t0; t < t1; ++t) {
+      const Rec nb = A.mir[t];
+      const double sx = per ? image_shift(nb.x, mx, I.len_x) : 0.0;
+      const double sy = per ? image_shift(nb.y, my, I.len_y) : 0.0;
+      const double xj = (nb.x + sx) - mx, yj = (nb.y + sy) - my;
+      const double r2 = sq_dist(xj, yj);
+      if (r2 > I.h_eff_sq) continue;
+      ++num;
+      const double vw = nb.vol * (I.w_norm * kshape_w<double>(A.kt, sqrt(r2) * I.inv_h));
+      double a[Q];
+      probe_basis<Q>(xj, yj, I.inv_h, a);
+      const double f = rec_field(nb, A.field);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const double vwa = vw * a[i];
+        b[0][i] += vwa * f;
+#pragma unroll
+        for (int j = 0; j <= i; ++j) M[i][j] += vwa * a[j];
+      }
+    }
+  });
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+#pragma unroll
+    for (int j = i + 1; j < Q; ++j) M[i][j] = M[j][i];
+  if (num < Q || lu_solve<Q, 1>(M, b) != 0) {
+    A.out[3 * k] = A.out[3 * k + 1] = A.out[3 * k + 2] = nan;
+    return;
+  }
+  A.out[3 * k] = b[0][0];
+  A.out[3 * k + 1] = b[0][1] * I.inv_h;
+  A.out[3 * k + 2] = b[0][2] * I.inv_h;
+}
+
 static inline int blocks(int n, int b) { return (n + b - 1) / b; }
 
 template <int G>
@@ -390,6 +495,18 @@ void launch_ghost(hipStream_t s, Device &d, int cur) {
     ghost_launch<2>(s, d.opt.wall_model, d.n_ext, a);
   else
     ghost_launch<4>(s, d.opt.wall_model, d.n_ext, a);
+}
+
+void launch_probe(hipStream_t s, Device &d, int cur, int inst, int field, int q, int n, const double *px,
+                  const double *py, double *out) {
+  ProbeArgs a{d.mir[cur], d.cell_start, d.inst, px, py, out, n, inst, field, d.opt.kernel_type};
+  const dim3 g(blocks(n, 64)), b(64);
+  if (q == 3)
+    hipLaunchKernelGGL(k_probe_b<3>, g, b, 0, s, a);
+  else if (q == 6)
+    hipLaunchKernelGGL(k_probe_b<6>, g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_probe_b<10>, g, b, 0, s, a);
 }
 
 void launch_vm_init(hipStream_t s, Device &d, int cur) {

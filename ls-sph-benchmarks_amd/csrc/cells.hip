@@ -29,6 +29,10 @@ __global__ void k_iota(int32_t *__restrict__ p, int n) {
   if (i < n) p[i] = i;
 }
 
+__global__ void k_add_i32(int32_t *__restrict__ p, int32_t v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p += v;
+}
+
 __global__ void k_fill_u64(unsigned long long *__restrict__ p, unsigned long long v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -434,6 +438,10 @@ __global__ void k_inverse_perm(int n, const int32_t *__restrict__ orig_of, int32
 
 // --- host launchers ---------------------------------------------------------
 static inline int blocks(int n, int b) { return (n + b - 1) / b; }
+
+void launch_add_i32(hipStream_t s, int32_t *p, int32_t v) {
+  hipLaunchKernelGGL(k_add_i32, dim3(1), dim3(64), 0, s, p, v);
+}
 
 void launch_fill_i32(hipStream_t s, int32_t *p, int32_t v, int n) {
   if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, s, p, v, n);

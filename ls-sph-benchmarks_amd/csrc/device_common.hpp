@@ -53,7 +53,8 @@ struct InstDev {
   double x_min, x_max, y_min, y_max;  // fluid box (walls beyond)
   int32_t nx, ny, cell_off, periodic;  // internal cell grid
   int32_t rnx, rny;                     // the reference's cell grid (cell_idx output)
-  int32_t pad0, pad1;
+  int32_t retired;  // isolation mode: set (by a vector store) when the instance is retired
+  int32_t pad1;
 };
 
 // Model options shared by every instance of a handle.
@@ -235,10 +236,11 @@ __device__ __forceinline__ int lu_solve(T (&a)[N][N], T (&b)[NR][N]) {
   return info;
 }
 
-// Instance constants through the constant address space: the memory is not
-// written while kernels run, so loads at a wave-uniform address (instance
-// index from readfirstlane) become scalar loads instead of per-lane vector
-// loads.
+// Instance constants through the constant address space: loads at a
+// wave-uniform address (instance index from readfirstlane) become scalar
+// loads instead of per-lane vector loads.  The only field written on the
+// device is `retired` (isolation mode), by vector stores that later kernels
+// see; a kernel running while it is set may read either value.
 __device__ __forceinline__ const InstDev *inst_const(const InstDev *p, int i) {
   using CInst = const __attribute__((address_space(4))) InstDev;
   return (const InstDev *)((CInst *)p + i);

@@ -366,12 +366,13 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_lsa_wave(LsaArgs A) {
   if (s >= A.n) return;
   WL &W = lds_all[w];
   const int32_t inf = A.sinfo[s];
-  if (info_ptype(inf) != 0) {
+  const InstDev &I = A.inst[info_inst(inf)];
+  // walls, and the particles of retired instances, move on with the records
+  if (info_ptype(inf) != 0 || I.retired) {
     if (lane == 0 && A.mode != kRhsOnly) A.mout[s] = A.mir[s];
     if (lane == 0 && A.spd) A.spd[s] = 0.0;
     return;
   }
-  const InstDev &I = A.inst[info_inst(inf)];
   const Rec me = A.mir[s];
   const int n = wave_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.counters, I, s, me, W, lane);
   double ih[5];
@@ -439,12 +440,12 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
   if (s >= A.n) return;
   WL &W = lds_all[w];
   const int32_t inf = A.sinfo[s];
-  if (info_ptype(inf) != 0) {
+  const int ii = info_inst(inf);
+  const InstDev &I = A.inst[ii];
+  if (info_ptype(inf) != 0 || I.retired) {
     if (lane == 0) A.mout[s] = A.mir[s];  // walls move on with the records
     return;
   }
-  const int ii = info_inst(inf);
-  const InstDev &I = A.inst[ii];
   const Rec me = A.mir[s];
   const int n = wave_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.mv.counters, I, s, me, W, lane);
   double ih[5];

@@ -18,9 +18,18 @@ enum {
   kStrayOverflow = 4,
   kSkinExceeded = 5,
   kWaveListOverflow = 6,  // a particle of the q = 20 kernels with too many neighbors
-  kFbDone = 7,            // blocks of k_lsa_fallback done (the last one clears the queue)
-  kNumCounters = 8
+  kFbDone = 7,            // blocks of the fallback kernel done (the last one clears the queue)
+  kStepNo = 8,            // isolation mode: 1-based number of the current time step
+  kNumCounters = 9
 };
+
+// Neighbor-list entries of the per-lane scan (and of the q = 20 kernels) are
+// 32 bits: the slot in the low kSlotBits bits, the periodic image code above
+// it.  lssph_create therefore refuses handles of more than kMaxParticles
+// particles (slots 0 .. kSlotMask).
+constexpr int kSlotBits = 27;
+constexpr int kSlotMask = (1 << kSlotBits) - 1;
+constexpr long long kMaxParticles = kSlotMask;
 
 // Incremental cell rebuild: particles whose new cell is outside the 3x3
 // stencil of their old cell are listed (old slot, new key) for the rebuild;
@@ -77,6 +86,11 @@ struct Device {
   int32_t *steady = nullptr;             // [n_inst]
   int32_t *counters = nullptr;           // [kNumCounters]
   unsigned long long *stats64 = nullptr;  // [kNumStats64]
+  // isolation mode (lssph_set_isolation): retired[k] = step during which
+  // instance k was retired, 0 while it runs, retired[n_inst + k] = reason;
+  // null when the mode is off
+  int32_t *retired = nullptr;             // [2 n_inst]
+  int32_t *retired_buf = nullptr;         // the allocation behind retired
   // host-side staging for downloads (device scratch, upload order)
   double *stage_f64 = nullptr;  // [8 * n_total]
   int32_t *stage_i32 = nullptr;  // [n_total]
@@ -90,6 +104,7 @@ struct Device {
 // cells.hip
 void launch_fill_i32(hipStream_t s, int32_t *p, int32_t v, int n);
 void launch_fill_u64(hipStream_t s, unsigned long long *p, unsigned long long v, int n);
+void launch_add_i32(hipStream_t s, int32_t *p, int32_t v);  // *p += v (one thread)
 int scan_tile_count(int n);
 void launch_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int32_t *tile_sums, int n,
                            int32_t *max_out);
@@ -113,10 +128,17 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
 // U_max from the speeds the last RK stage stored (valid while slots are unchanged).
 void launch_umax_spd(hipStream_t s, Device &d);
 void launch_umax(hipStream_t s, Device &d, int cur);
+// The neighbor lists of the pair kernels as their walk visits them (upload
+// indices, cap entries per fluid particle) and the scan path of each.
+void launch_pair_lists(hipStream_t s, Device &d, int cur, int32_t *counts, int32_t *list, int32_t *path, int cap);
 void launch_dfmax(hipStream_t s, Device &d, int cur);
 void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys);
 void launch_ghost(hipStream_t s, Device &d, int cur);
 void launch_vm_init(hipStream_t s, Device &d, int cur);
+// Type-B LS fit of field (0 u, 1 v, 2 rho, 3 pre, 4 tem) at n probe points of
+// one instance, q = 3, 6 or 10: out[3k..3k+2] = value, d/dx, d/dy.
+void launch_probe(hipStream_t s, Device &d, int cur, int inst, int field, int q, int n, const double *px,
+                  const double *py, double *out);
 void launch_neighbor_count(hipStream_t s, Device &d, int cur, int32_t *counts_orig);
 void launch_neighbor_fill(hipStream_t s, Device &d, int cur, const int64_t *offsets_orig, int32_t *list,
                           int64_t *sort_key);

@@ -182,15 +182,25 @@ __device__ __forceinline__ double sym5(const R (&M)[15], int i, int c) {
 // (x += M^-1 (b - M x), the residual formed in FP64 against the FP32 sums).
 // The right-hand sides are processed one at a time and the sums stay FP32 in
 // registers, which keeps the solve's register footprint near that of the
-// pair loop.  Returns false when a pivot is not positive or the pivot ratio
-// max(d)/min(d) of the basis5 system -- a cheap lower bound of its
-// cond_2(M) -- exceeds cond_max; the caller then re-solves in full FP64.
-// X holds the coefficients of the basis5 terms.
+// pair loop.  X holds the coefficients of the basis5 terms.
+//
+// Returns false -- the caller then redoes the particle in full FP64 -- when
+// a pivot is not positive or when the 1-norm condition number of the basis5
+// system, kappa_1 = ||M||_1 ||M^-1||_1, exceeds cond_max (or is not finite).
+// kappa_1 is exact here, not an estimate that can undershoot: M^-1 =
+// L^-T D^-1 L^-1 is formed from the factors (about 130 FP64 operations per
+// particle; ||M||_1 is taken on the FP32 sums).  The basis5 system is
+// D M D with D = diag(1, 1, 1/2, 1/2, 1),
+// so its entries and those of its inverse are the basis5s ones times
+// D_i D_j and 1 / (D_i D_j).  (The pivot ratio max(d) / min(d) that this
+// replaces is a lower bound of cond_2 and misses correlated basis columns:
+// on one-sided supports it falls short of kappa_1 by up to 12x.)
+__device__ __forceinline__ double basis5_weight(int i) { return (i == 2 || i == 3) ? 0.5 : 1.0; }
+
 template <int NF>
 __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, double (&X)[NF][5],
                                                   double cond_max) {
   double L[5][5], d[5], dinv[5];
-  double dmax = 0.0, dmin = 1.0e300;
   bool ok = true;
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
@@ -200,10 +210,6 @@ __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, do
     ok = ok && (dj > 0.0);
     d[j] = dj;
     dinv[j] = 1.0 / dj;
-    // pivots of the basis5 system: those of the squares are 4x larger here
-    const double dj5 = (j == 2 || j == 3) ? 0.25 * dj : dj;
-    dmax = fmax(dmax, dj5);
-    dmin = fmin(dmin, dj5);
 #pragma unroll
     for (int i = j + 1; i < 5; ++i) {
       double t = sym5(S.M, i, j);
@@ -212,7 +218,46 @@ __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, do
       L[i][j] = t * dinv[j];
     }
   }
-  if (!ok || dmax > cond_max * dmin) return false;
+  if (!ok) return false;
+  {
+    // ||M||_1 on the FP32 sums; ||M^-1||_1 one column at a time: column j
+    // solves M y = e_j with the factors (L y' = e_j, y' /= d, L^T y = y'),
+    // so only five doubles of the inverse are live at once
+    float nrm_m = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      float col = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        const float e = i >= j ? S.M[i * (i + 1) / 2 + j] : S.M[j * (j + 1) / 2 + i];
+        col += fabsf(e) * float(basis5_weight(i));
+      }
+      nrm_m = fmaxf(nrm_m, col * float(basis5_weight(j)));
+    }
+    double nrm_inv = 0.0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      double y[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        double t = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = j; k < i; ++k) t -= L[i][k] * y[k];
+        y[i] = (i < j) ? 0.0 : t;
+      }
+#pragma unroll
+      for (int i = j; i < 5; ++i) y[i] *= dinv[i];
+#pragma unroll
+      for (int i = 3; i >= 0; --i)
+#pragma unroll
+        for (int k = i + 1; k < 5; ++k) y[i] -= L[k][i] * y[k];
+      double col = 0.0;
+#pragma unroll
+      for (int i = 0; i < 5; ++i) col += fabs(y[i]) / basis5_weight(i);
+      nrm_inv = fmax(nrm_inv, col / basis5_weight(j));
+    }
+    if (!(double(nrm_m) * nrm_inv <= cond_max)) return false;
+  }
   auto solve = [&](double (&y)[5]) {
 #pragma unroll
     for (int i = 1; i < 5; ++i)
@@ -295,7 +340,7 @@ __device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int
   LsSums<double, NF, Q> S;
   S.zero();
   walk_neighbors(A.mir, A.cstart, I, lc, s, me, L,
-                 [&](const Rec &nb, double dx, double dy) {
+                 [&](const Rec &nb, double dx, double dy, int) {
                    const double r = sqrt(sq_dist(dx, dy));
                    const double w = I.w_norm * kshape_w<double, KT>(kt, r * I.inv_h);
                    double a[Q];
@@ -325,7 +370,7 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
   LsSums<float, NF> S;
   S.zero();
   walk_neighbors<true>(A.mir, A.cstart, I, lc, s, me, L,
-                 [&](const Rec &nb, double dx, double dy) {
+                 [&](const Rec &nb, double dx, double dy, int) {
                    const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
                    const float q = fast_sqrtf(xs * xs + ys * ys);
                    // the common factor w_norm of M and b cancels in the solve
@@ -345,9 +390,13 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
 
 // RHS of the governing equations (LSSPH_A_mod.f90:246-268) and the stage
 // update (RK2_mod.f90:77-107, RK4_mod.f90:216-286) of physical particle p.
+// Evaluated without FMA contraction, like the reference's arithmetic
+// (baseline x86-64 has no FMA), so that every kernel that inlines it -- the
+// operators and their FP64 redo -- rounds it identically.
 template <int CLS, bool DELTA>
 __device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p, int s, const Rec &me,
                                        const double (&d)[Layout<CLS, DELTA>::NF][5]) {
+#pragma clang fp contract(off)
   using L = Layout<CLS, DELTA>;
   double r_u, r_v = 0.0, r_rho = 0.0, r_tem = 0.0;
   if constexpr (CLS == 0) {
@@ -490,6 +539,9 @@ struct PstArgs {
   const InstDev *__restrict__ inst;
   const unsigned long long *__restrict__ umax;
   unsigned long long *stats64;
+  int32_t *fb_list;   // slots queued for the FP64 redo (mixed path)
+  int32_t *fb_count;
+  int32_t *fb_done;
   MoveOut mv;
   int n, kt, list_len;
   int staged;  // wave-staged candidate scan allowed

@@ -18,8 +18,9 @@
 //   FP64 path: LU with partial pivoting (DGESV semantics).
 //   Mixed path: FP32 pair sums on FP64-formed offsets and differences, FP64
 //   LDL^T + one refinement step; particles whose LDL^T pivots are not
-//   positive or whose pivot ratio exceeds cond_max are queued and redone by
-//   the FP64 path in a second, list-driven launch.
+//   positive or whose moment matrix has a 1-norm condition number above
+//   cond_max are queued and redone by the FP64 path in a second,
+//   list-driven launch (k_lsa_fallback, k_pst_fallback).
 #include "ls_common.hpp"
 
 namespace lssph {
@@ -76,18 +77,17 @@ __global__ void __launch_bounds__(256)
   const int s = xcd_block() * B + tid;
   if (s >= A.n) return;
   const int32_t inf = A.sinfo[s];
-  if (info_ptype(inf) != 0) {
-    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];  // walls move on with the records
+  // walls (and the particles of retired instances) move on with the records
+  auto keep = [&] {
+    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];
     if (A.spd) A.spd[s] = 0.0;
+  };
+  if (info_ptype(inf) != 0) {
+    keep();
     return;
   }
-  // A wave of one instance (the common case) reads the instance constants
-  // with scalar loads; a wave that spans instances, with per-lane loads.
-  const int ii = info_inst(inf), ii0 = __builtin_amdgcn_readfirstlane(ii);
-  if (__ballot(ii != ii0) == 0ull)
-    lsa_particle<CLS, MIXED, DELTA, KT, Q>(A, *inst_const(A.inst, ii0), s, tid, lds);
-  else
-    lsa_particle<CLS, MIXED, DELTA, KT, Q>(A, A.inst[ii], s, tid, lds);
+  for_instance(A.inst, inf, [&](const InstDev &I, int) { lsa_particle<CLS, MIXED, DELTA, KT, Q>(A, I, s, tid, lds); },
+               keep);
 }
 
 // FP64 redo of the slots the mixed kernel queued (direct neighbor walk).
@@ -143,7 +143,7 @@ __device__ __forceinline__ void csph_particle(const LsaArgs &A, const InstDev &I
   double du_dx = 0.0, dv_dy = 0.0, dp_dx = 0.0, dp_dy = 0.0;
   double lap_u = 0.0, lap_v = 0.0, lap_t = 0.0, lap_rho = 0.0;
   walk_neighbors(A.mir, A.cstart, I, lc, s, me, nl,
-                 [&](const Rec &nb, double dx, double dy) {
+                 [&](const Rec &nb, double dx, double dy, int) {
                    const double r = sqrt(sq_dist(dx, dy));
                    if (!(r > 0.0)) return;  // the particle itself
                    const double x_ij = -dx, y_ij = -dy;
@@ -192,16 +192,15 @@ __global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
   const int s = xcd_block() * B + tid;
   if (s >= A.n) return;
   const int32_t inf = A.sinfo[s];
-  if (info_ptype(inf) != 0) {
+  auto keep = [&] {
     if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];
     if (A.spd) A.spd[s] = 0.0;
+  };
+  if (info_ptype(inf) != 0) {
+    keep();
     return;
   }
-  const int ii = info_inst(inf), ii0 = __builtin_amdgcn_readfirstlane(ii);
-  if (__ballot(ii != ii0) == 0ull)
-    csph_particle<CLS, DELTA, SUM>(A, *inst_const(A.inst, ii0), s, tid, lds);
-  else
-    csph_particle<CLS, DELTA, SUM>(A, A.inst[ii], s, tid, lds);
+  for_instance(A.inst, inf, [&](const InstDev &I, int) { csph_particle<CLS, DELTA, SUM>(A, I, s, tid, lds); }, keep);
 }
 
 // ---------------------------------------------------------------------------
@@ -209,7 +208,9 @@ __global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
 // ---------------------------------------------------------------------------
 
 // Q = 5 for LS-SPH (SPH_MODEL 3), 9 and 14 for SPH_MODEL 4-5 (FP64), 2 for
-// the classical models.
+// the classical models.  tid >= 0: the neighbors come from the LDS list the
+// scan fills; tid < 0 (the FP64 redo of k_pst_fallback): from a direct walk
+// over the stencil, in the same order.
 template <bool T, bool MIXED, int KT, int Q>
 __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I, int ii, int s, int tid,
                                              int32_t *lds) {
@@ -219,10 +220,14 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
   const int lc = local_cell(I, me.x, me.y);
   const int kt = A.kt;
   // list entries, the particle itself included
-  NbList nl = make_list(lds, A.list_len, tid);
-  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl, A.staged != 0);
+  NbList nl = no_list();
+  if (tid >= 0) {
+    nl = make_list(lds, A.list_len, tid);
+    build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl, A.staged != 0);
+    if (nl.cnt > A.list_len) atomicAdd(&A.stats64[kOverflows], 1ull);
+  }
+  // a particle redone in FP64 was queued with cnt - 1 >= Q
   const int cnt = nl.cnt;
-  if (cnt > A.list_len) atomicAdd(&A.stats64[kOverflows], 1ull);
 
   LsSums<R, NF, Q> S;
   S.zero();
@@ -231,7 +236,7 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
   // the deep walk for the mixed Wendland kernel without temperature (the
   // others would spill under the 128-register cap)
   walk_neighbors<MIXED && !T && KT == 3>(A.mir, A.cstart, I, lc, s, me, nl,
-                 [&](const Rec &nb, double dx, double dy) {
+                 [&](const Rec &nb, double dx, double dy, int) {
                    R df[NF];
                    df[0] = R(nb.u - me.u);
                    df[1] = R(nb.v - me.v);
@@ -284,6 +289,11 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
                    }
                  });
   if constexpr (MIXED) S.complete();
+  {
+  // The epilogue is evaluated without FMA contraction, like the
+  // reference's (baseline x86-64 has no FMA), so that every kernel that
+  // inlines it -- k_pst and its FP64 redo -- rounds it identically.
+#pragma clang fp contract(off)
   // shift vector and its clamp (shift_interpolate_mod.f90:333-346)
   const double umax = __longlong_as_double((long long)A.umax[ii]);
   const double coe = I.pst_c * umax * I.dt * I.h;
@@ -296,16 +306,18 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
   // LS correction of u, v, rho (, tem) to the shifted position (:352-415)
   double X[NF][Q];
   if (cnt - 1 >= Q) {  // shift_interpolate_mod.f90:240 (num_neighbors >= q)
-    bool solved = false;
     if constexpr (MIXED) {
-      solved = ldlt_solve_refine<NF>(S, X, I.cond_max);
-      if (!solved) atomicAdd(&A.stats64[kFallbacks], 1ull);
-    }
-    if (!solved) {
+      if (!ldlt_solve_refine<NF>(S, X, I.cond_max)) {
+        // the whole particle -- shift sums and fit -- is redone in FP64 by
+        // k_pst_fallback
+        const int k = atomicAdd(A.fb_count, 1);
+        A.fb_list[k] = s;
+        return;
+      }
+    } else {
       double M[Q][Q];
       unpack(S, M, X);
       lu_solve<Q, NF>(M, X);
-      if constexpr (MIXED) unscale_squares<NF>(X);  // the mixed sums are of the basis5s basis
     }
   } else {
 #pragma unroll
@@ -323,18 +335,23 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
     for (int i = 0; i < Q; ++i) t += a[i] * X[f][i];
     acc[f] = t;
   }
-  Rec out = me;
+  // the particle's own record is read again rather than kept live across
+  // the solve (as in k_lsa), which lowers the peak register count
+  __asm__ volatile("" ::: "memory");
+  const Rec me2 = A.mir[s];
+  Rec out = me2;
   const bool per = I.periodic;
-  out.x = wrap_pos(me.x + dxs, I.len_x, per);
-  out.y = wrap_pos(me.y + dys, I.len_y, per);
-  out.u = me.u + acc[0];
-  out.v = me.v + acc[1];
-  out.rho = me.rho + acc[2];
-  if constexpr (T) out.tem = me.tem + acc[3];
+  out.x = wrap_pos(me2.x + dxs, I.len_x, per);
+  out.y = wrap_pos(me2.y + dys, I.len_y, per);
+  out.u = me2.u + acc[0];
+  out.v = me2.v + acc[1];
+  out.rho = me2.rho + acc[2];
+  if constexpr (T) out.tem = me2.tem + acc[3];
   out.pre = I.c_sq * (out.rho - I.rho_ref);
   out.vol = I.sp_mass / out.rho;
   A.mout[s] = out;
   record_move(A.mv, I, s, out.x, out.y);
+  }
 }
 
 
@@ -349,15 +366,36 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED 
   const int s = xcd_block() * B + tid;
   if (s >= A.n) return;
   const int32_t inf = A.sinfo[s];
+  auto keep = [&] { A.mout[s] = A.mir[s]; };  // walls move on with the records
   if (info_ptype(inf) != 0) {
-    A.mout[s] = A.mir[s];  // walls move on with the records
+    keep();
     return;
   }
-  const int ii = info_inst(inf), ii0 = __builtin_amdgcn_readfirstlane(ii);
-  if (__ballot(ii != ii0) == 0ull)
-    pst_particle<T, MIXED, KT, Q>(A, *inst_const(A.inst, ii0), ii0, s, tid, lds);
-  else
-    pst_particle<T, MIXED, KT, Q>(A, A.inst[ii], ii, s, tid, lds);
+  for_instance(A.inst, inf, [&](const InstDev &I, int ii) { pst_particle<T, MIXED, KT, Q>(A, I, ii, s, tid, lds); },
+               keep);
+}
+
+// FP64 redo of the particles the mixed shifting kernel queued: shift sums,
+// moment sums and solve in FP64 over a direct stencil walk (the order of the
+// FP64 kernel's list), so a redone particle equals the FP64 path's result.
+template <bool T, int KT>
+__global__ void __launch_bounds__(256) k_pst_fallback(PstArgs A) {
+  const int count = *A.fb_count;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const int s = A.fb_list[i];
+    const int ii = info_inst(A.sinfo[s]);
+    pst_particle<T, false, KT, 5>(A, A.inst[ii], ii, s, -1, nullptr);
+    atomicAdd(&A.stats64[kFallbacks], 1ull);
+  }
+  // the last block to finish empties the queue (as in k_lsa_fallback)
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(A.fb_done, 1) == (int)gridDim.x - 1) {
+      *A.fb_count = 0;
+      *A.fb_done = 0;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -428,6 +466,45 @@ __global__ void __launch_bounds__(256) k_dfmax(const Rec *__restrict__ rec, cons
 }
 
 // ---------------------------------------------------------------------------
+// Validation: the neighbor lists the pair kernels walk, as they walk them.
+// The same scan (build_list) and walk (walk_neighbors, four-deep) as k_lsa
+// and k_pst, with the same launch shape and LDS layout; the walk records
+// the upload index of each neighbor it visits (-1 - index for a visit at a
+// distance above h_eff, which never happens in a correct walk) and the path
+// the particle took: 0 per-lane scan, 1 staged scan row by row, 2 staged
+// scan with the positions kept in LDS, 3 list overflow (direct walk).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void pair_list_particle(const LsaArgs &A, const InstDev &I, int s, int tid, int32_t *lds,
+                                                   int32_t *counts, int32_t *list, int32_t *path, int cap) {
+  const Rec me = A.mir[s];
+  const int lc = local_cell(I, me.x, me.y);
+  NbList nl = make_list(lds, A.list_len, tid);
+  build_list(A.mir, A.cstart, I, lc, me.x, me.y, I.h_eff_sq, nl, A.staged != 0);
+  const int p = A.orig_of[A.items[s]];
+  int k = 0;
+  walk_neighbors<true>(A.mir, A.cstart, I, lc, s, me, nl, [&](const Rec &, double dx, double dy, int t) {
+    if (t == s) return;  // the particle itself, a zero offset in the list
+    const int o = A.orig_of[A.items[t]];
+    if (k < cap) list[(size_t)p * cap + k] = sq_dist(dx, dy) <= I.h_eff_sq ? o : -1 - o;
+    ++k;
+  });
+  counts[p] = k;
+  path[p] = nl.cnt > nl.list_len ? 3 : (nl.base >= 0 ? (nl.pos ? 2 : 1) : 0);
+}
+
+__global__ void __launch_bounds__(256) k_pair_lists(LsaArgs A, int32_t *counts, int32_t *list, int32_t *path,
+                                                    int cap) {
+  extern __shared__ int32_t lds[];
+  const int tid = threadIdx.x, B = blockDim.x;
+  const int s = xcd_block() * B + tid;
+  if (s >= A.n) return;
+  const int32_t inf = A.sinfo[s];
+  if (info_ptype(inf) != 0) return;
+  for_instance(A.inst, inf,
+               [&](const InstDev &I, int) { pair_list_particle(A, I, s, tid, lds, counts, list, path, cap); }, [] {});
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 // LDS of the neighbor lists: list_len 32-bit entries per thread plus the
@@ -486,7 +563,7 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
   a.fb_list = d.fb_list;
   a.fb_count = d.counters + kFbCount;
   a.fb_done = d.counters + kFbDone;
-  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters, d.xbuild, guard, keys};
+  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters, d.xbuild, guard, keys, d.retired, d.sinfo, d.n_inst, d.inst};
   a.n = d.n_total;
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;
@@ -548,6 +625,24 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
     delta ? lsa_dispatch<2, true>(s, a, d.block, lds, mixed, kt) : lsa_dispatch<2, false>(s, a, d.block, lds, mixed, kt);
 }
 
+void launch_pair_lists(hipStream_t s, Device &d, int cur, int32_t *counts, int32_t *list, int32_t *path,
+                       int cap) {
+  LsaArgs a{};
+  a.mir = d.mir[cur];
+  a.sinfo = d.sinfo;
+  a.items = d.items;
+  a.orig_of = d.orig_of;
+  a.cstart = d.cell_start;
+  a.inst = d.inst;
+  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters, d.xbuild, 0, 0, d.retired, d.sinfo, d.n_inst, d.inst};
+  a.n = d.n_total;
+  a.kt = d.opt.kernel_type;
+  a.list_len = d.list_len;
+  a.staged = d.staged_scan;
+  hipLaunchKernelGGL(k_pair_lists, dim3(xcd_grid(a.n, d.block)), dim3(d.block), list_lds_bytes(d.list_len, d.block),
+                     s, a, counts, list, path, cap);
+}
+
 void launch_umax(hipStream_t s, Device &d, int cur) {
   launch_fill_u64(s, d.umax, 0ull, d.n_inst);
   hipLaunchKernelGGL(k_umax, dim3(d.red_blocks, d.n_inst), dim3(256), 0, s, d.mir[cur], d.sinfo, d.cell_start,
@@ -569,10 +664,13 @@ void launch_dfmax(hipStream_t s, Device &d, int cur) {
 template <bool T, bool MIXED>
 static void pst_launch(hipStream_t s, const PstArgs &a, int block, size_t lds, int kt) {
   const dim3 g(xcd_grid(a.n, block)), b(block);
-  if (kt == 3)
+  if (kt == 3) {
     hipLaunchKernelGGL((k_pst<T, MIXED, 3, 5>), g, b, lds, s, a);
-  else
+    if (MIXED) hipLaunchKernelGGL((k_pst_fallback<T, 3>), dim3(64), dim3(128), 0, s, a);
+  } else {
     hipLaunchKernelGGL((k_pst<T, MIXED, 0, 5>), g, b, lds, s, a);
+    if (MIXED) hipLaunchKernelGGL((k_pst_fallback<T, 0>), dim3(64), dim3(128), 0, s, a);
+  }
 }
 
 void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
@@ -586,7 +684,10 @@ void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
   a.inst = d.inst;
   a.umax = d.umax;
   a.stats64 = d.stats64;
-  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters, d.xbuild, guard, keys};
+  a.fb_list = d.fb_list;
+  a.fb_count = d.counters + kFbCount;
+  a.fb_done = d.counters + kFbDone;
+  a.mv = MoveOut{d.kslot, d.strays, d.nstray, d.counters, d.xbuild, guard, keys, d.retired, d.sinfo, d.n_inst, d.inst};
   a.n = d.n_total;
   a.kt = d.opt.kernel_type;
   a.list_len = d.list_len;

@@ -14,10 +14,6 @@
 
 namespace lssph {
 
-// LDS list entries: slot in the low bits, periodic image code above.
-constexpr int kSlotBits = 27;
-constexpr int kSlotMask = (1 << kSlotBits) - 1;
-
 // A lane's neighbor list in the block's LDS.  Each wavefront owns
 // list_len + 1 rows of 256 bytes (4 * (list_len + 1) * blockDim bytes per
 // block in all), and lane l's entry k sits at byte 256 k + 4 l of its
@@ -360,8 +356,8 @@ __device__ __forceinline__ void build_list(const Rec *__restrict__ mir, const in
   }
 }
 
-// Calls f(nb_record, dx, dy) for every neighbor within h_eff of slot s, in
-// stencil order, by a walk over the candidates of the cell list.
+// Calls f(nb_record, dx, dy, slot) for every neighbor within h_eff of slot
+// s, in stencil order, by a walk over the candidates of the cell list.
 template <typename F>
 __device__ __forceinline__ void walk_stencil(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
                                              const InstDev &I, int lc, int s, const Rec &me, F &&f) {
@@ -373,7 +369,7 @@ __device__ __forceinline__ void walk_stencil(const Rec *__restrict__ mir, const 
       const double sx = per ? image_shift(xy.x, me.x, I.len_x) : 0.0;
       const double sy = per ? image_shift(xy.y, me.y, I.len_y) : 0.0;
       const double dx = (xy.x + sx) - me.x, dy = (xy.y + sy) - me.y;
-      if (sq_dist(dx, dy) <= I.h_eff_sq) f(mir[t], dx, dy);
+      if (sq_dist(dx, dy) <= I.h_eff_sq) f(mir[t], dx, dy, t);
     }
   });
 }
@@ -393,7 +389,7 @@ __device__ __forceinline__ void list_loop(int n, B &&body) {
   }
 }
 
-// Calls f(nb_record, dx, dy) for every accepted neighbor of slot s in
+// Calls f(nb_record, dx, dy, slot) for every accepted neighbor of slot s in
 // summation order: from the LDS list filled by scan_candidates or
 // scan_staged, which holds the particle itself as a pair with dx = dy = 0,
 // or (the list overflowed) by a walk over the candidates, which skips it.
@@ -416,20 +412,23 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
       // (e >> 14) * 16 = (e >> 10) & 0x30, as i < 1024 + 8
       const unsigned d = (unsigned)(L.rowd >> ((e >> 10) & 0x30u)) & 0xffffu;
       const double2 p = stage[i];
-      const Rec nb = mir[L.base + (int)(i + d)];
-      f(nb, p.x - me.x, p.y - me.y);
+      const int t = L.base + (int)(i + d);
+      const Rec nb = mir[t];
+      f(nb, p.x - me.x, p.y - me.y, t);
     });
   } else if (L.base >= 0) {
     const char *l16 = L.wbase + 2 * L.lane;
     list_loop<DEEP>(cnt, [&](int k) {
-      const Rec nb = mir[L.base + *reinterpret_cast<const uint16_t *>(l16 + (k << 7))];
-      f(nb, nb.x - me.x, nb.y - me.y);
+      const int t = L.base + *reinterpret_cast<const uint16_t *>(l16 + (k << 7));
+      const Rec nb = mir[t];
+      f(nb, nb.x - me.x, nb.y - me.y, t);
     });
   } else if (!wave_wraps(I, lc)) {
     const char *l32 = L.wbase + 4 * L.lane;
     for (int k = 0; k < cnt; ++k) {
-      const Rec nb = mir[*reinterpret_cast<const int32_t *>(l32 + (k << 8)) & kSlotMask];
-      f(nb, nb.x - me.x, nb.y - me.y);
+      const int t = *reinterpret_cast<const int32_t *>(l32 + (k << 8)) & kSlotMask;
+      const Rec nb = mir[t];
+      f(nb, nb.x - me.x, nb.y - me.y, t);
     }
   } else {
     const char *l32 = L.wbase + 4 * L.lane;
@@ -439,7 +438,7 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
       const Rec nb = mir[t];
       const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
       const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
-      f(nb, dx, dy);
+      f(nb, dx, dy, t);
     }
   }
 }
@@ -448,15 +447,18 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
 // (floor(x / width)), so that the 3x3 stencil of a cell list built at one
 // position still holds every pair within h_eff after each particle has moved
 // by up to skin / 2.  Periodic boxes clamp the upper edge.  Returns false
-// outside the grid.
+// outside the grid, also for a non-finite position (the range test is made
+// on the floating-point cell coordinate, before any conversion).
 __device__ __forceinline__ bool fluid_cell_xy(const InstDev &I, double x, double y, int &cx, int &cy) {
-  cx = (int)floor(x * I.inv_cw_x);
-  cy = (int)floor(y * I.inv_cw_y);
+  double fx = floor(x * I.inv_cw_x), fy = floor(y * I.inv_cw_y);
   if (I.periodic) {
-    cx = (cx == I.nx) ? I.nx - 1 : cx;
-    cy = (cy == I.ny) ? I.ny - 1 : cy;
+    fx = (fx == (double)I.nx) ? (double)(I.nx - 1) : fx;
+    fy = (fy == (double)I.ny) ? (double)(I.ny - 1) : fy;
   }
-  return cx >= 0 && cx < I.nx && cy >= 0 && cy < I.ny;
+  const bool in = fx >= 0.0 && fx < (double)I.nx && fy >= 0.0 && fy < (double)I.ny;
+  cx = in ? (int)fx : -1;
+  cy = in ? (int)fy : -1;
+  return in;
 }
 
 // Local cell index on the internal grid, -1 outside.
@@ -523,6 +525,10 @@ __device__ __forceinline__ void ref_cell_xy(const InstDev &I, double x, double y
 //           and clamped into the grid like the full build does; a move that
 //           leaves the 3x3 stencil of the old cell is queued on the stray
 //           list.
+// A non-finite position fails both checks.  In isolation mode (retired
+// non-null) such a move retires the particle's instance instead of raising
+// an error: the particle keeps its cell key, and from the next pass on the
+// kernels leave the instance's records as they are.
 struct MoveOut {
   int32_t *kslot;
   int32_t *strays;
@@ -530,23 +536,74 @@ struct MoveOut {
   int32_t *counters;
   const double *xbuild;  // [2 * n] position at the last cell build, per slot
   int guard, keys;
+  int32_t *retired;      // [2 n_inst], isolation mode only: step, then reason
+  const int32_t *sinfo;  // instance of a slot (read when retiring)
+  int n_inst;
+  InstDev *inst_rw;      // the instance constants, whose `retired` flag a retirement sets
 };
 
+// Runs run(I, ii) for the particle of info word inf with its instance's
+// constants, or keep() when the instance has been retired (isolation mode).
+// A wave of one instance (the common case) reads the constants with scalar
+// loads; a wave that spans instances, with per-lane loads.
+template <typename F, typename K>
+__device__ __forceinline__ void for_instance(const InstDev *inst, int32_t inf, F &&run, K &&keep) {
+  const int ii = info_inst(inf), ii0 = __builtin_amdgcn_readfirstlane(ii);
+  if (__ballot(ii != ii0) == 0ull) {
+    const InstDev &I = *inst_const(inst, ii0);
+    if (I.retired)
+      keep();
+    else
+      run(I, ii0);
+  } else {
+    const InstDev &I = inst[ii];
+    if (I.retired)
+      keep();
+    else
+      run(I, ii);
+  }
+}
+
+// Marks the instance of slot s retired during the current step, with the
+// cause: 1 (LSSPH_RETIRED_SKIN) a move beyond the skin, 2
+// (LSSPH_RETIRED_ESCAPED) a position outside the grid or not finite.  Every
+// thread that retires the instance in one pass stores the same step (later
+// passes skip the instance), and the reason is one of the causes seen.
+__device__ __forceinline__ void retire_instance(const MoveOut &M, int s, int reason) {
+  const int ii = info_inst(M.sinfo[s]);
+  M.retired[ii] = max(M.counters[kStepNo], 1);
+  M.retired[M.n_inst + ii] = reason;
+  M.inst_rw[ii].retired = 1;
+}
+
 __device__ __forceinline__ void record_move(const MoveOut &M, const InstDev &I, int s, double x, double y) {
+  // the common case takes one branch; the rare outcomes (a move beyond the
+  // skin, a position outside the grid) are handled after it
+  int reason = 0;
   if (M.guard) {
     double dx = x - M.xbuild[2 * s], dy = y - M.xbuild[2 * s + 1];
     if (I.periodic) {
       dx = dx > 0.5 * I.len_x ? dx - I.len_x : (dx < -0.5 * I.len_x ? dx + I.len_x : dx);
       dy = dy > 0.5 * I.len_y ? dy - I.len_y : (dy < -0.5 * I.len_y ? dy + I.len_y : dy);
     }
-    if (dx * dx + dy * dy > I.skin_half_sq) M.counters[kSkinExceeded] = 1;
+    if (!(dx * dx + dy * dy <= I.skin_half_sq)) reason = LSSPH_RETIRED_SKIN;
   }
-  if (!M.keys) return;
-  int cx, cy;
-  if (!fluid_cell_xy(I, x, y, cx, cy)) {
-    atomicAdd(&M.counters[kEscaped], 1);
+  int cx = 0, cy = 0;
+  if (M.keys && !fluid_cell_xy(I, x, y, cx, cy)) {
+    reason = LSSPH_RETIRED_ESCAPED;
     cx = cy = 0;
   }
+  if (reason != 0) {
+    if (M.retired) {
+      retire_instance(M, s, reason);  // the particle keeps its cell key
+      return;
+    }
+    if (reason == LSSPH_RETIRED_SKIN)
+      M.counters[kSkinExceeded] = 1;
+    else
+      atomicAdd(&M.counters[kEscaped], 1);  // clamped into the grid like the full build does
+  }
+  if (!M.keys) return;
   const int nx = I.nx, ny = I.ny;
   const int oc = M.kslot[s] - I.cell_off;
   int ddx = cx - oc % nx, ddy = cy - oc / nx;

@@ -172,6 +172,29 @@ module lssph_binding
             type(c_ptr), value :: h
             type(lssph_kernel_times_t), intent(out) :: out
         end function
+        integer(c_int32_t) function lssph_pair_lists(h, cap, counts, list, path) bind(C, name="lssph_pair_lists")
+            import :: c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: cap
+            integer(c_int32_t), intent(out) :: counts(*), list(*), path(*)
+        end function
+        integer(c_int32_t) function lssph_probe(h, inst, field, q, n, x, y, out) bind(C, name="lssph_probe")
+            import :: c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: inst, field, q, n
+            real(c_double), intent(in) :: x(*), y(*)
+            real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int32_t) function lssph_set_isolation(h, on) bind(C, name="lssph_set_isolation")
+            import :: c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: on
+        end function
+        integer(c_int32_t) function lssph_instance_status(h, retired, reason) bind(C, name="lssph_instance_status")
+            import :: c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), intent(out) :: retired(*), reason(*)
+        end function
     end interface
 
 contains

@@ -51,7 +51,8 @@ EXPORTS = ("lssph_create", "lssph_destroy", "lssph_last_error", "lssph_upload", 
            "lssph_download_rhs", "lssph_rk2", "lssph_rk4", "lssph_pst", "lssph_step",
            "lssph_check_steady_state", "lssph_download_vm", "lssph_download_cell_index",
            "lssph_neighbors", "lssph_stats", "lssph_reset_stats", "lssph_synchronize",
-           "lssph_set_tuning", "lssph_set_skin", "lssph_set_scan", "lssph_stream", "lssph_profile", "lssph_kernel_times")
+           "lssph_set_tuning", "lssph_set_skin", "lssph_set_scan", "lssph_stream", "lssph_profile", "lssph_kernel_times",
+           "lssph_pair_lists", "lssph_probe", "lssph_set_isolation", "lssph_instance_status")
 
 
 def load_library(path: Path | None = None):
@@ -89,6 +90,10 @@ def load_library(path: Path | None = None):
     lib.lssph_stream.argtypes = [vp]
     lib.lssph_profile.argtypes = [vp, i32]
     lib.lssph_kernel_times.argtypes = [vp, ctypes.POINTER(KernelTimes)]
+    lib.lssph_pair_lists.argtypes = [vp, i32, _I, _I, _I]
+    lib.lssph_probe.argtypes = [vp, i32, i32, i32, i32, _D, _D, _D]
+    lib.lssph_set_isolation.argtypes = [vp, i32]
+    lib.lssph_instance_status.argtypes = [vp, _I, _I]
     for name in EXPORTS:
         if name not in ("lssph_last_error", "lssph_stream"):
             getattr(lib, name).restype = i32
@@ -261,6 +266,44 @@ class Simulation:
         lst = np.zeros(max(total.value, 1), np.int32)
         self._check(self.lib.lssph_neighbors(self.h, _ip(counts), _ip(lst), total.value, ctypes.byref(total)))
         return counts, lst[: total.value]
+
+    def pair_lists(self, cap: int = 128):
+        """The pair kernels' neighbor lists as their walk visits them:
+        (counts, list[n_int, cap] of upload indices, scan path per particle)."""
+        counts = np.zeros(self.n_int, np.int32)
+        path = np.zeros(self.n_int, np.int32)
+        lst = np.zeros((self.n_int, cap), np.int32)
+        self._check(self.lib.lssph_pair_lists(self.h, int(cap), _ip(counts), _ip(lst), _ip(path)))
+        return counts, lst, path
+
+    PROBE_FIELDS = {"u": 0, "v": 1, "rho": 2, "pre": 3, "tem": 4}
+
+    def probe(self, x, y, field: str = "tem", q: int = 6, inst: int = 0) -> np.ndarray:
+        """Type-B LS fit of a field at probe points (upload coordinates):
+        an (n, 3) array of value, d/dx, d/dy."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        if x.shape != y.shape or x.ndim != 1:
+            raise ValueError("probe: x and y must be 1-D arrays of one length")
+        out = np.zeros((x.size, 3))
+        self._check(self.lib.lssph_probe(self.h, int(inst), self.PROBE_FIELDS[field], int(q), int(x.size),
+                                         _dp(x), _dp(y), _dp(out)))
+        return out
+
+    def set_isolation(self, on: bool = True):
+        """Ensembles: retire a diverged instance instead of failing the handle."""
+        self._check(self.lib.lssph_set_isolation(self.h, 1 if on else 0))
+
+    RETIRED_SKIN = 1     # lssph.h LSSPH_RETIRED_SKIN
+    RETIRED_ESCAPED = 2  # lssph.h LSSPH_RETIRED_ESCAPED
+
+    def instance_status(self, reasons: bool = False):
+        """Per instance: 0 while it runs, else the step during which it was
+        retired; with reasons=True also the cause (RETIRED_SKIN, RETIRED_ESCAPED)."""
+        out = np.zeros(self.n_inst, np.int32)
+        why = np.zeros(self.n_inst, np.int32)
+        self._check(self.lib.lssph_instance_status(self.h, _ip(out), _ip(why)))
+        return (out, why) if reasons else out
 
     def stats(self) -> Stats:
         s = Stats()

@@ -110,3 +110,25 @@ def test_simulation_rejects_counts_that_do_not_match_the_instances():
     param, st = workloads.taylor_green_periodic(n=16, re=100.0, rk=2)
     with pytest.raises(ValueError, match="parameter sets"):
         engine.Simulation(param, [st.num_int, st.num_int], [0])
+
+
+def test_create_rejects_more_particles_than_the_list_entries_address_without_touching_the_gpu():
+    """Neighbor-list entries keep a particle's slot in 27 bits, so a handle
+    holds at most 2^27 - 1 particles; 2^27 is refused before any device
+    allocation (a 2^27-particle closed box would otherwise read wrong
+    neighbors silently)."""
+    lib = engine.load_library()
+    p = _params()
+    h = ctypes.c_void_p()
+    ni = (ctypes.c_int32 * 1)((1 << 27) - 4096)
+    ne = (ctypes.c_int32 * 1)(4096)
+    assert lib.lssph_create(ctypes.byref(p), 1, ni, ne, ctypes.byref(h)) == -1
+    assert b"27-bit" in lib.lssph_last_error(None)
+    assert not h.value
+    # the same count split over an ensemble is refused as well
+    arr = (lcfg.Params * 2)(p, p)
+    ni2 = (ctypes.c_int32 * 2)(1 << 26, 1 << 26)
+    ne2 = (ctypes.c_int32 * 2)(0, 0)
+    assert lib.lssph_create(arr, 2, ni2, ne2, ctypes.byref(h)) == -1
+    assert b"27-bit" in lib.lssph_last_error(None)
+    assert not h.value

@@ -1,0 +1,179 @@
+"""The three at-scale configurations at their full size and length, mixed
+precision against the all-FP64 GPU path on the same input:
+
+  2048^2 periodic Taylor-Green, jitter 0.2 dx, Re 1000, RK4 + PST, 500 steps
+  64 lid-driven cavities (sizes {32..256}^2, Re 100..1000 from seed 2030),
+      RK2 + PST, one graph replay per step, 2000 steps
+  1024^2 bottom-heated Boussinesq convection (Ra from seed 2031), WALL_MODEL
+      3, RK4 + PST, 5000 steps, with the mean Nusselt numbers of both walls
+
+The Taylor-Green run is compared particle by particle (bounds about ten
+times the measured differences; the L2 errors against the analytic decay
+must agree within 1 %).  The cavity and convection runs amplify any
+difference until the particle arrangement is scrambled, so they are
+compared through the quantities the reference's analysis reports, against
+an FP64 run from a slightly perturbed start as the yardstick.
+"""
+
+import numpy as np
+import pytest
+
+from lssph import workloads
+from lssph.engine import Simulation, simulation_from_state, split_states
+
+pytestmark = pytest.mark.gpu
+
+
+def _diff(a, b, n, fields, periodic=False):
+    out = {}
+    for k in fields:
+        d = a[k][:n] - b[k][:n]
+        if periodic and k in ("x", "y"):
+            d = (d + 0.5) % 1.0 - 0.5
+        out[k] = (float(np.max(np.abs(d))), float(np.sqrt(np.mean(d * d))))
+    return out
+
+
+def _check(diff, bounds):
+    for k, (mx, l2) in diff.items():
+        bmax, bl2 = bounds[k]
+        assert mx <= bmax and l2 <= bl2, (k, mx, l2, bmax, bl2)
+
+
+# (max, L2) bounds per field, about 10x the measured differences
+C3_BOUNDS = {"x": (2e-8, 1e-9), "y": (2e-8, 1e-9), "u": (1e-7, 5e-9), "v": (1e-7, 5e-9),
+             "rho": (2e-8, 2e-9), "pre": (2e-6, 2e-7)}
+
+
+def test_taylor_green_2048_mixed_vs_fp64_500_steps():
+    param, st = workloads.taylor_green_periodic(n=2048, re=1000.0, rk=4, jitter=0.2, seed=2029)
+    n = st.num_int
+    out, stats = {}, {}
+    for prec in (1, 0):
+        print(f"[gpu] Taylor-Green 2048^2 precision={prec}: 500 steps", flush=True)
+        with simulation_from_state(param, st, precision=prec) as s:
+            s.step(500)
+            out[prec] = s.download()
+            stats[prec] = s.stats()
+    t = 500 * param.dt
+    l2 = {p: workloads.tg_l2_error(param, out[p], t, n) for p in (0, 1)}
+    d = _diff(out[1], out[0], n, ("x", "y", "u", "v", "rho", "pre"), periodic=True)
+    print(f"[gpu] mixed vs fp64 (max, L2): {d}; TG L2 {l2}; "
+          f"fallbacks {stats[1].fp64_fallbacks} overflows {stats[1].list_overflows}", flush=True)
+    assert abs(l2[1] - l2[0]) <= 0.01 * l2[0], l2
+    _check(d, C3_BOUNDS)
+    assert stats[1].fp64_fallbacks == 0 and stats[1].list_overflows == 0
+    assert stats[0].list_overflows == 0
+
+
+def _perturbed(st, dx, seed, eps=1e-7):
+    """The state with every fluid position moved by up to eps * dx (uniform,
+    SplitMix64 seed): the yardstick run for flows in which any difference
+    grows until the particle arrangement is scrambled."""
+    from lssph.particles import splitmix64_uniform
+    st = st.copy()
+    r = splitmix64_uniform(seed, 2 * st.num_int)
+    st.x[:st.num_int] += eps * dx * (2.0 * r[0::2] - 1.0)
+    st.y[:st.num_int] += eps * dx * (2.0 * r[1::2] - 1.0)
+    return st
+
+
+# instances of workloads.cavity_ensemble(64) that the reference's scheme
+# cannot run for 2000 steps (tests/test_oracle_ensemble.py)
+DIVERGING_CAVITIES = [5, 20, 45, 48, 49]
+
+
+def test_cavity_ensemble_64_mixed_vs_fp64_2000_steps():
+    """Five of the 64 cavities (32^2 at Re 535-750, cell Reynolds numbers of
+    17-23) diverge between steps 1700 and 2000 in the reference's own
+    scheme: the compiled reference's velocity maximum grows from 0.94 to 4.7
+    between steps 1600 and 1800 on instance 5 before it crashes, and the
+    oracle diverges on the same five instances.  With isolation on, every
+    run retires exactly those and the other 59 run their 2000 steps.
+
+    Over 2000 steps the shifted particle arrangement of a cavity amplifies
+    any difference by ~1e11 (a summation-order change grows from 1e-16 to
+    1e-5), so particles are compared through the quantities the reference's
+    cavity analysis reports (cavity_flow.py:74-106): V_rms and the
+    centerline profiles u(x = L/2, y), v(x, y = L/2) from type-B fits.  The
+    yardstick is an FP64 run from a start perturbed by 1e-7 dx: measured,
+    the mixed run's profile differences have the median 7.3e-3 and the
+    maximum 0.15 (of the lid velocity) over the 59 instances, the perturbed
+    run's 7.3e-3 and 0.20."""
+    from lssph import analysis
+    prof, vr, status = {}, {}, {}
+    for name, prec, pert in (("mixed", 1, False), ("fp64", 0, False), ("fp64_perturbed", 0, True)):
+        params, states, meta = workloads.cavity_ensemble(count=64, precision=prec)
+        if pert:
+            states = [_perturbed(st, p.dx, 77 + k) for k, (st, p) in enumerate(zip(states, params))]
+        ni = [s.num_int for s in states]
+        ne = [s.num_ext for s in states]
+        print(f"[gpu] 64-cavity ensemble ({sum(ni)} fluid), {name}: 2000 steps", flush=True)
+        with Simulation(params, ni, ne, precision=prec) as sim:
+            sim.set_isolation(True)
+            sim.upload(states)
+            sim.init_cell()
+            sim.ghost()
+            sim.step(2000)
+            status[name] = sim.instance_status()
+            out = split_states(sim.download(), ni, ne)
+            prof[name] = [analysis.cavity_profiles(sim, params[k], inst=k) for k in range(64)]
+            vr[name] = np.array([analysis.vrms(out[k], ni[k]) for k in range(64)])
+            stats = sim.stats()
+            assert stats.list_overflows == 0
+    for name in status:
+        assert sorted(np.nonzero(status[name])[0].tolist()) == DIVERGING_CAVITIES, (name, status[name])
+    run = [k for k in range(64) if k not in DIVERGING_CAVITIES]
+
+    def prof_diff(a, b):
+        return np.array([max(np.max(np.abs(prof[a][k][0][1] - prof[b][k][0][1])),
+                             np.max(np.abs(prof[a][k][1][1] - prof[b][k][1][1]))) for k in run])
+
+    dm, dp = prof_diff("mixed", "fp64"), prof_diff("fp64_perturbed", "fp64")
+    vm = np.abs(vr["mixed"][run] - vr["fp64"][run])
+    vp = np.abs(vr["fp64_perturbed"][run] - vr["fp64"][run])
+    print(f"[gpu] centerline profiles, max |difference| per instance: mixed median {np.median(dm):.3e} max "
+          f"{dm.max():.3e}; perturbed FP64 median {np.median(dp):.3e} max {dp.max():.3e}", flush=True)
+    print(f"[gpu] V_rms |difference|: mixed median {np.median(vm):.3e} max {vm.max():.3e}; perturbed FP64 median "
+          f"{np.median(vp):.3e} max {vp.max():.3e}", flush=True)
+    assert np.all(np.isfinite(dm))
+    assert np.median(dm) <= 2.0 * np.median(dp) and dm.max() <= 2.0 * dp.max()
+    assert np.median(vm) <= 2.0 * np.median(vp) and vm.max() <= 2.0 * vp.max()
+
+
+def test_boussinesq_1024_mixed_vs_fp64_5000_steps():
+    """A local instability at the left wall (x ~ 0, y ~ 0.8) amplifies any
+    difference: a summation-order change grows from 7e-21 at step 20 to 5e-4
+    at step 5000.  So the mixed run is held to the yardstick of an FP64 run
+    from a start perturbed by 1e-7 dx, in the mean Nusselt numbers of both
+    walls (boussinesq_conv.py:80-108) and in the velocity field.  Measured:
+    Nu differs from the FP64 run's by 5e-7 and 6.5e-6 (mixed) and 3.9e-6 and
+    5.0e-6 (perturbed); the velocity L2 differences are 5.0e-4 and 4.1e-4.
+    No particle needs the FP64 redo (the moment matrices near the walls
+    include the ghost particles and stay well conditioned)."""
+    from lssph import analysis
+    from lssph.particles import splitmix64_uniform
+    ra = (1e4, 1e5, 1e6)[min(int(splitmix64_uniform(2031, 1)[0] * 3), 2)]
+    out, stats, nu = {}, {}, {}
+    for name, prec, pert in (("mixed", 1, False), ("fp64", 0, False), ("fp64_perturbed", 0, True)):
+        param, st = workloads.boussinesq(n=1024, ra=ra, jitter=0.2, seed=2030, precision=prec)
+        if pert:
+            st = _perturbed(st, param.dx, 99)
+        print(f"[gpu] Boussinesq 1024^2 Ra {ra:g} {name}: 5000 steps", flush=True)
+        with simulation_from_state(param, st, precision=prec) as s:
+            s.ghost()
+            s.step(5000)
+            out[name] = s.download()
+            stats[name] = s.stats()
+            nu[name] = np.array(analysis.nusselt(s, param)[:2])
+    n = st.num_int
+    for p in out.values():
+        assert np.all(np.isfinite(p["tem"][:n])) and np.all(np.isfinite(p["u"][:n]))
+    dnu_m = np.abs(nu["mixed"] - nu["fp64"])
+    dnu_p = np.abs(nu["fp64_perturbed"] - nu["fp64"])
+    l2 = {k: float(np.sqrt(np.mean((out[k]["u"][:n] - out["fp64"]["u"][:n]) ** 2))) for k in ("mixed", "fp64_perturbed")}
+    print(f"[gpu] Nu (bottom, top): {({k: v.tolist() for k, v in nu.items()})}; |dNu| mixed {dnu_m}, perturbed "
+          f"{dnu_p}; u L2 difference {l2}; fallbacks {stats['mixed'].fp64_fallbacks}", flush=True)
+    assert np.all(dnu_m <= 3.0 * dnu_p.max() + 1e-7) and np.all(dnu_m <= 1e-4)
+    assert l2["mixed"] <= 3.0 * l2["fp64_perturbed"]
+    assert stats["mixed"].fp64_fallbacks == 0 and stats["mixed"].list_overflows == 0
