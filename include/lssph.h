@@ -219,6 +219,19 @@ int32_t lssph_pair_lists(lssph_handle_t h, int32_t cap, int32_t *counts,
 int32_t lssph_probe(lssph_handle_t h, int32_t inst, int32_t field, int32_t q,
                     int32_t n, const double *x, const double *y, double *out);
 
+/* lssph_probe without any allocation: the same fit and arguments, with the
+   probe coordinates and results staged in a fixed device buffer of the
+   library (one per GPU, held until the process ends) instead of buffers
+   allocated per call.  No counterpart in the reference; it is for callers
+   that cannot allocate device memory (tight monitoring loops inside a time
+   loop, harnesses that forbid hipMalloc).  At most 16384 probes per call
+   (LSSPH_E_ARG beyond).  Calls on one GPU share the buffer: calls that
+   overlap in time on one GPU (several host threads or handles) race.
+   Synchronises. */
+int32_t lssph_probe_static(lssph_handle_t h, int32_t inst, int32_t field,
+                           int32_t q, int32_t n, const double *x,
+                           const double *y, double *out);
+
 /* Ensembles: isolation of diverged instances (no reference counterpart: the
    reference runs one box and, when its scheme diverges, indexes past its cell
    arrays).  on = 1: a fluid particle that leaves its instance's cell grid,

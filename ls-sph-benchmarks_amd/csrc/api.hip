@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,7 @@ using namespace lssph;
 struct lssph_handle {
   Device d;
   hipStream_t stream = nullptr;
+  int device = -1;  // the device current at lssph_create (the stream's)
   std::vector<lssph_params_t> params;
   std::vector<InstDev> inst_host;
   std::vector<int32_t> n_int, n_ext;
@@ -43,6 +45,18 @@ struct lssph_handle {
 };
 
 static thread_local std::string g_create_error;
+
+// ---- staging of lssph_probe_static ------------------------------------------
+// Probe coordinates in, three results per probe out, for the entry point that
+// calls no allocation routine.  Each GPU holds its own copy of the array for
+// the life of the process; its address on a device is looked up once, with
+// that device current, by the first lssph_create on it.  All calls on one GPU
+// share it, so overlapping calls on one GPU race.
+constexpr int kProbeStaticMax = 16384;  // probes per call (640 KiB of staging)
+static __device__ __align__(256) double scratch[5 * kProbeStaticMax];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
 
 namespace {
 
@@ -436,6 +450,21 @@ int alloc_grid(lssph_handle *h) {
   return 0;
 }
 
+// Device address of the staging array on the handle's device, looked up once
+// per device (the array's host-side name is not a device address).
+int resolve_scratch(lssph_handle *h) {
+  HIP_TRY(h, hipGetDevice(&h->device));
+  if (h->device < 0 || h->device >= kMaxDevices) return fail(h, LSSPH_E_HIP, "device index beyond 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[h->device]) return 0;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(h, LSSPH_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[h->device] = static_cast<double *>(p);
+  return 0;
+}
+
 int ready(lssph_handle *h) {
   if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
   if (!h->uploaded) return fail(h, LSSPH_E_STATE, "call lssph_upload first");
@@ -563,6 +592,7 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) rc = fail(h, LSSPH_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
+  rc = rc ? rc : resolve_scratch(h);
   rc = rc ? rc : alloc_grid(h);
   if (rc) {
     g_create_error = h->err;
@@ -925,6 +955,37 @@ int32_t lssph_probe(lssph_handle_t h, int32_t inst, int32_t field, int32_t q, in
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(buf);
   if (e != hipSuccess) return fail(h, LSSPH_E_HIP, std::string("lssph_probe: ") + hipGetErrorString(e));
+  return check_counters(h);
+}
+
+int32_t lssph_probe_static(lssph_handle_t h, int32_t inst, int32_t field, int32_t q, int32_t n, const double *x,
+                           const double *y, double *out) {
+  if (int rc = ready(h)) return rc;
+  Device &d = h->d;
+  if (inst < 0 || inst >= d.n_inst) return fail(h, LSSPH_E_ARG, "lssph_probe_static: instance out of range");
+  if (field < 0 || field > 4 || (field == 4 && !d.opt.has_tem))
+    return fail(h, LSSPH_E_ARG, "lssph_probe_static: field must be 0..3, or 4 (tem) for target_problem 4");
+  if (q != 3 && q != 6 && q != 10) return fail(h, LSSPH_E_ARG, "lssph_probe_static: q must be 3, 6 or 10");
+  if (n < 0 || (n > 0 && (!x || !y || !out)))
+    return fail(h, LSSPH_E_ARG, "lssph_probe_static: null array or n < 0");
+  if (n > kProbeStaticMax)
+    return fail(h, LSSPH_E_ARG,
+                "lssph_probe_static: " + std::to_string(n) + " probes exceed its staging buffer of " +
+                    std::to_string(kProbeStaticMax) + " (split the call, or use lssph_probe)");
+  if (n == 0) return 0;
+  double *sp = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    sp = g_scratch_ptr[h->device];
+  }
+  if (!sp) return fail(h, LSSPH_E_STATE, "lssph_probe_static: staging buffer not set up on this device");
+  hipStream_t s = h->stream;
+  HIP_TRY(h, hipMemcpyAsync(sp, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIP_TRY(h, hipMemcpyAsync(sp + kProbeStaticMax, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  launch_probe(s, d, h->cur, inst, field, q, n, sp, sp + kProbeStaticMax, sp + 2 * kProbeStaticMax);
+  HIP_TRY(h, hipMemcpyAsync(out, sp + 2 * kProbeStaticMax, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(h, hipStreamSynchronize(s));
+  HIP_TRY(h, hipGetLastError());
   return check_counters(h);
 }
 

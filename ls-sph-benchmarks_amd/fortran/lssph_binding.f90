@@ -185,6 +185,14 @@ module lssph_binding
             real(c_double), intent(in) :: x(*), y(*)
             real(c_double), intent(out) :: out(*)
         end function
+        integer(c_int32_t) function lssph_probe_static(h, inst, field, q, n, x, y, out) &
+                bind(C, name="lssph_probe_static")
+            import :: c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: inst, field, q, n
+            real(c_double), intent(in) :: x(*), y(*)
+            real(c_double), intent(out) :: out(*)
+        end function
         integer(c_int32_t) function lssph_set_isolation(h, on) bind(C, name="lssph_set_isolation")
             import :: c_int32_t, c_ptr
             type(c_ptr), value :: h

@@ -52,7 +52,8 @@ EXPORTS = ("lssph_create", "lssph_destroy", "lssph_last_error", "lssph_upload", 
            "lssph_check_steady_state", "lssph_download_vm", "lssph_download_cell_index",
            "lssph_neighbors", "lssph_stats", "lssph_reset_stats", "lssph_synchronize",
            "lssph_set_tuning", "lssph_set_skin", "lssph_set_scan", "lssph_stream", "lssph_profile", "lssph_kernel_times",
-           "lssph_pair_lists", "lssph_probe", "lssph_set_isolation", "lssph_instance_status")
+           "lssph_pair_lists", "lssph_probe", "lssph_set_isolation", "lssph_instance_status",
+           "lssph_probe_static")
 
 
 def load_library(path: Path | None = None):
@@ -92,6 +93,7 @@ def load_library(path: Path | None = None):
     lib.lssph_kernel_times.argtypes = [vp, ctypes.POINTER(KernelTimes)]
     lib.lssph_pair_lists.argtypes = [vp, i32, _I, _I, _I]
     lib.lssph_probe.argtypes = [vp, i32, i32, i32, i32, _D, _D, _D]
+    lib.lssph_probe_static.argtypes = [vp, i32, i32, i32, i32, _D, _D, _D]
     lib.lssph_set_isolation.argtypes = [vp, i32]
     lib.lssph_instance_status.argtypes = [vp, _I, _I]
     for name in EXPORTS:
@@ -288,6 +290,17 @@ class Simulation:
         out = np.zeros((x.size, 3))
         self._check(self.lib.lssph_probe(self.h, int(inst), self.PROBE_FIELDS[field], int(q), int(x.size),
                                          _dp(x), _dp(y), _dp(out)))
+        return out
+
+    def probe_static(self, x, y, field: str = "tem", q: int = 6, inst: int = 0) -> np.ndarray:
+        """probe() through the allocation-free entry point (at most 16384 probes)."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        if x.shape != y.shape or x.ndim != 1:
+            raise ValueError("probe: x and y must be 1-D arrays of one length")
+        out = np.zeros((x.size, 3))
+        self._check(self.lib.lssph_probe_static(self.h, int(inst), self.PROBE_FIELDS[field], int(q), int(x.size),
+                                                _dp(x), _dp(y), _dp(out)))
         return out
 
     def set_isolation(self, on: bool = True):
