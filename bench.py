@@ -101,28 +101,58 @@ def run_gpu(param, st, precision: int, steps: int, warmup: int) -> dict:
     return {"ms": ms, "kt": kt, "stats": stats, "state": out, "l2": l2, "upload_ms": up_ms, "download_ms": down_ms}
 
 
+def physical_cores() -> int | None:
+    """Physical cores of this host from lscpu (distinct core, socket pairs)."""
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    pairs = {ln.strip() for ln in out.splitlines() if ln and not ln.startswith("#")}
+    return len(pairs) or None
+
+
+def _ref_timing(case: str, threads: int, steps: int, warmup: int) -> float | None:
+    """particle-steps/s of the reference Fortran built for `case` (oracle/_ref),
+    timed by its loop timer (main.f90:110,162) over `steps` steps after
+    `warmup` untimed ones."""
+    exe = REPO / "oracle" / "_ref" / case / "ref_harness"
+    if not exe.exists():
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        try:
+            r = subprocess.run([str(exe), td, str(steps), "0", str(threads), str(warmup)], capture_output=True,
+                               text=True, timeout=240, cwd=td,
+                               env=dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores"))
+        except (OSError, subprocess.TimeoutExpired) as e:
+            print(f"[bench] reference run {case} failed: {e}", flush=True)
+            return None
+    m = re.search(r"particle_steps_per_s\s+(\S+)", r.stdout)
+    if r.returncode != 0 or not m:
+        print(f"[bench] reference run {case} failed (rc={r.returncode}): {r.stderr[-300:]}", flush=True)
+        return None
+    return float(m.group(1))
+
+
 def cpu_baseline(threads: int) -> dict | None:
-    """The reference Fortran (oracle/_ref/tg2048_bench, built from the
-    reference sources with MKL) timed on this host; falls back to the C port
-    of the oracle when that executable cannot run here."""
-    exe = REPO / "oracle" / "_ref" / "tg2048_bench" / "ref_harness"
-    if exe.exists():
-        with tempfile.TemporaryDirectory() as td:
-            try:
-                print(f"[bench] cpu baseline: reference Fortran, {threads} threads", flush=True)
-                r = subprocess.run([str(exe), td, "2", "0", str(threads)], capture_output=True, text=True,
-                                   timeout=240, cwd=td,
-                                   env=dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores"))
-                m = re.search(r"particle_steps_per_s\s+(\S+)", r.stdout)
-                if r.returncode == 0 and m:
-                    return {"value": float(m.group(1)), "unit": "particle-steps/s", "cores": threads,
-                            "kind": "reference",
-                            "sample": "reference Fortran+MKL, 2048x2048 Taylor-Green in the closed box with "
-                                      "free-slip ghost walls, jitter 0.2 dx, RK4+PST, 2 timed steps "
-                                      "(loop only, main.f90:110,162 timer)"}
-                print(f"[bench] reference run failed (rc={r.returncode}): {r.stderr[-300:]}", flush=True)
-            except (OSError, subprocess.TimeoutExpired) as e:
-                print(f"[bench] reference run failed: {e}", flush=True)
+    """The reference Fortran (oracle/_ref, built from the reference sources
+    with MKL) timed on this host, as the measurement plan asks: 1 warm-up and
+    5 timed steps of the loop, on all the cores this process may use (2048^2)
+    and on 1 core (512^2, so the single-core sample stays near 10 s).  Falls
+    back to the C port of the oracle when those executables cannot run here."""
+    phys = physical_cores()
+    print(f"[bench] cpu baseline: reference Fortran, {threads} threads (host: {phys} physical cores)", flush=True)
+    v_all = _ref_timing("tg2048_bench", threads, 5, 1)
+    if v_all is not None:
+        v_one = _ref_timing("tg512_bench", 1, 5, 1)
+        return {"value": v_all, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
+                "host_physical_cores": phys,
+                "sample": "reference Fortran+MKL, 2048x2048 Taylor-Green in the closed box with free-slip ghost "
+                          "walls (the reference has no periodic box), jitter 0.2 dx, RK4+PST, 1 warm-up + 5 timed "
+                          f"steps (loop only, main.f90:110,162 timer), {threads} OpenMP threads = this process's "
+                          "CPU share",
+                "single_core": None if v_one is None else {
+                    "value": v_one, "unit": "particle-steps/s", "cores": 1,
+                    "sample": "the same at 512x512 on 1 thread, 1 warm-up + 5 timed steps"}}
     try:
         from oracle.pyoracle import OracleSim
         param, st = workloads.taylor_green_periodic(n=512, re=1000.0, rk=4, jitter=0.2, seed=2029)
@@ -131,13 +161,25 @@ def cpu_baseline(threads: int) -> dict | None:
         o.init(st)
         o.step(1)
         t0 = time.perf_counter()
-        o.step(3)
+        o.step(5)
         dt = time.perf_counter() - t0
-        return {"value": st.num_int * 3 / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-                "sample": "C port of the reference loops, 512x512 periodic Taylor-Green, RK4+PST, 3 steps"}
+        return {"value": st.num_int * 5 / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                "host_physical_cores": phys,
+                "sample": "C port of the reference loops, 512x512 periodic Taylor-Green, RK4+PST, "
+                          "1 warm-up + 5 timed steps"}
     except Exception as e:  # noqa: BLE001
         print(f"[bench] port baseline failed: {e}", flush=True)
         return None
+
+
+def cpu_threads() -> int:
+    """The threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when set (the GPU box sets it to its per-GPU share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
 TRAFFIC_PROFILE = REPO / "profiles" / "kernels_tg2048_mixed_hbm_traffic.txt"
@@ -209,7 +251,7 @@ def model_bench(args) -> None:
         "tg_l2_error": r["l2"]}), flush=True)
 
 
-def _timed_run(make, steps: int, warmup: int, graph: bool) -> dict:
+def _timed_run(make, steps: int, warmup: int, graph: bool, post=None) -> dict:
     """Steps of a ready Simulation bracketed by HIP events on its stream
     (graph replay or eager calls), then a short eager run with per-kernel
     events for the breakdown (after the state is downloaded)."""
@@ -224,7 +266,9 @@ def _timed_run(make, steps: int, warmup: int, graph: bool) -> dict:
     e1.record(sim.stream)
     ms = e0.elapsed_ms(e1)
     stats = sim.stats()
+    status = sim.instance_status(reasons=True) if sim.n_inst > 1 else None
     out = sim.download()
+    extra = post(sim, out) if post else None
     k = max(1, min(steps, 20))
     sim.kernel_times()
     sim.profile(True)
@@ -232,22 +276,28 @@ def _timed_run(make, steps: int, warmup: int, graph: bool) -> dict:
     sim.profile(False)
     kt = sim.kernel_times()
     sim.close()
-    return {"ms": ms, "stats": stats, "state": out,
+    return {"ms": ms, "stats": stats, "state": out, "status": status, "post": extra,
             "breakdown": {g: round(getattr(kt, g + "_ms") / k, 4) for g in ("lsa", "pst", "cells", "ghost", "other")}}
 
 
 def ensemble_bench(args) -> None:
     """64 independent lid-driven cavities in one handle (every launch covers
     all of them), RK2 + PST, mixed precision, one HIP-graph replay per step,
-    with the launch settings picked by lssph.tune.autotune on this workload."""
+    2000 steps by default, with the launch settings picked by
+    lssph.tune.autotune on this workload.  Isolation is on: the five 32^2
+    cavities at Re 535-750 diverge near step 1800 in the reference's own
+    scheme and are retired there; `value` counts the particle-steps of the
+    instances while they run."""
     from lssph import tune
     from lssph.engine import Simulation
+    steps = args.steps or 2000
     params, states, meta = workloads.cavity_ensemble(count=64, precision=1)
     ni = [st.num_int for st in states]
     ne = [st.num_ext for st in states]
 
     def make(skin: float = 0.08):
         sim = Simulation(params, ni, ne, precision=1, skin=skin)
+        sim.set_isolation(True)
         sim.upload(states)
         sim.init_cell()
         sim.ghost()
@@ -263,19 +313,30 @@ def ensemble_bench(args) -> None:
         sim.set_tuning(block=b.block, list_len=b.list_len)
         return sim
 
-    r = _timed_run(make_best, args.steps, args.warmup, graph=True)
+    print(f"[bench] ensemble: {steps} graph-replayed steps", flush=True)
+    r = _timed_run(make_best, steps, args.warmup, graph=True)
+    retired, why = r["status"]
+    # retired[k] counts the steps from the warmup on; instance k ran until
+    # the step before
+    run_steps = [steps if retired[k] == 0 else max(0, min(steps, int(retired[k]) - 1 - args.warmup))
+                 for k in range(len(states))]
+    active = sum(n * s_ for n, s_ in zip(ni, run_steps))
     sizes = {}
     for m in meta:
         sizes[f"{m['n']}x{m['n']}"] = sizes.get(f"{m['n']}x{m['n']}", 0) + 1
     print(json.dumps({
-        "metric": "ensemble particle-steps/s (fluid particles, all instances)",
-        "value": round(sum(ni) * args.steps / (r["ms"] / 1e3), 1), "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["ms"] / args.steps, 4),
+        "metric": "ensemble particle-steps/s (fluid particles of the running instances)",
+        "value": round(active / (r["ms"] / 1e3), 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(r["ms"] / steps, 4),
         "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "64 lid-driven cavities, sizes drawn from {32,64,128,256}^2 and Re log-uniform "
                                "in [100, 1000] (SplitMix64 seed 2030), 3 wall layers, lid velocity 1, WALL_MODEL 2, "
-                               "LS q=5, RK2 + PST every step, mixed precision, one graph replay per step",
+                               "LS q=5, RK2 + PST every step, mixed precision, one graph replay per step, "
+                               "diverged instances retired (isolation)",
                    "instances": len(states), "sizes": sizes, "fluid_particles": sum(ni), "wall_particles": sum(ne)},
+        "retired": {str(k): {"step": int(retired[k]), "reason": "skin" if why[k] == 1 else "escaped",
+                             "n": meta[k]["n"], "re": round(meta[k]["re"], 1)}
+                    for k in range(len(states)) if retired[k]},
         "tuning": tuned.as_dict(),
         "ms_per_step_by_kernel_eager": r["breakdown"],
         "stats": {"fp64_fallbacks": int(r["stats"].fp64_fallbacks),
@@ -285,37 +346,53 @@ def ensemble_bench(args) -> None:
 def boussinesq_bench(args) -> None:
     """Bottom-heated convection, 1024 x 1024 fluid particles with no-slip
     walls, 4 coupled fields, WALL_MODEL 3 (6 x 6 truncated supports), RK4 +
-    PST: the mixed path against the FP64 path on the same input, per field,
-    with the mixed path's FP64 re-solves counted."""
+    PST, 5000 steps by default: the mixed path against the FP64 path on the
+    same input, and an FP64 run from a start perturbed by 1e-7 dx as the
+    yardstick of how far the flow amplifies any difference; the mean Nusselt
+    numbers of both walls (boussinesq_conv.py:80-108) of each."""
+    from lssph import analysis
     from lssph.particles import splitmix64_uniform
     ra = (1e4, 1e5, 1e6)[min(int(splitmix64_uniform(2031, 1)[0] * 3), 2)]
     n = args.n if args.n != 2048 else 1024
+    steps = args.steps or 5000
     runs = {}
-    for prec in (1, 0):
+    for name, prec, pert in (("mixed", 1, False), ("fp64", 0, False), ("fp64_perturbed", 0, True)):
         param, st = workloads.boussinesq(n=n, ra=ra, jitter=0.2, seed=2030, precision=prec)
+        if pert:
+            r = splitmix64_uniform(99, 2 * st.num_int)
+            st = st.copy()
+            st.x[:st.num_int] += 1e-7 * param.dx * (2.0 * r[0::2] - 1.0)
+            st.y[:st.num_int] += 1e-7 * param.dx * (2.0 * r[1::2] - 1.0)
 
         def make(param=param, st=st, prec=prec):
             sim = simulation_from_state(param, st, precision=prec)
             sim.ghost()
             return sim
 
-        print(f"[bench] boussinesq {n}x{n} Ra={ra:g} precision={'mixed' if prec else 'fp64'}", flush=True)
-        runs[prec] = _timed_run(make, args.steps, args.warmup, graph=True)
+        print(f"[bench] boussinesq {n}x{n} Ra={ra:g} {name}: {steps} steps", flush=True)
+        runs[name] = _timed_run(make, steps, args.warmup, graph=True,
+                                post=lambda sim, out, param=param: analysis.nusselt(sim, param)[:2])
     n_part = st.num_int
-    m, f = runs[1], runs[0]
+    m, f = runs["mixed"], runs["fp64"]
+    nu = {k: [round(v, 10) for v in r_["post"]] for k, r_ in runs.items()}
     print(json.dumps({
         "metric": "particle-steps/s, Boussinesq convection, mixed precision",
-        "value": round(n_part * args.steps / (m["ms"] / 1e3), 1), "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(m["ms"] / args.steps, 4),
+        "value": round(n_part * steps / (m["ms"] / 1e3), 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(m["ms"] / steps, 4),
         "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"bottom-heated Boussinesq convection {n}x{n}, jitter 0.2dx, Ra {ra:g}, Pr 0.71, "
                                "no-slip walls, T_bottom 1, T_top 0, adiabatic sides, WALL_MODEL 3, LS q=5, "
                                "RK4 + PST every step",
                    "particles": n_part, "walls": st.num_ext},
-        "fp64": {"value": round(n_part * args.steps / (f["ms"] / 1e3), 1),
-                 "ms_per_step": round(f["ms"] / args.steps, 4)},
+        "fp64": {"value": round(n_part * steps / (f["ms"] / 1e3), 1),
+                 "ms_per_step": round(f["ms"] / steps, 4)},
+        "nusselt_bottom_top": nu,
+        "nusselt_abs_diff_vs_fp64": {k: [abs(a - b) for a, b in zip(nu[k], nu["fp64"])]
+                                     for k in ("mixed", "fp64_perturbed")},
         "mixed_vs_fp64": field_diff(m["state"], f["state"], n_part,
                                     fields=("x", "y", "u", "v", "rho", "pre", "tem"), periodic=False),
+        "fp64_perturbed_vs_fp64": field_diff(runs["fp64_perturbed"]["state"], f["state"], n_part,
+                                             fields=("u", "v", "tem"), periodic=False),
         "ms_per_step_by_kernel_eager": m["breakdown"],
         "stats": {"fp64_fallbacks": int(m["stats"].fp64_fallbacks),
                   "list_overflows": int(m["stats"].list_overflows)}}), flush=True)
@@ -324,12 +401,13 @@ def boussinesq_bench(args) -> None:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps; 0: 500 (Taylor-Green), 2000 (ensemble), 5000 (Boussinesq)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=2048, help="particles per side")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: the threads this process may use")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--sph-model", type=int, default=3,
@@ -352,6 +430,7 @@ def main():
     if args.workload == "boussinesq":
         boussinesq_bench(args)
         return
+    args.steps = args.steps or 500
     if args.sph_model != 3:
         model_bench(args)
         return
@@ -403,7 +482,7 @@ def main():
         result["tg_l2_error"]["fp64"] = fp64["l2"]
         result["tg_l2_error"]["rel_diff"] = abs(mixed["l2"] - fp64["l2"]) / fp64["l2"]
         result["mixed_vs_fp64"] = field_diff(mixed["state"], fp64["state"], n_part)
-    result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_threads)
+    result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_threads or cpu_threads())
     print(json.dumps(result), flush=True)
 
 

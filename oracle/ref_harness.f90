@@ -4,13 +4,14 @@
 ! the reference's own modules; the reference sources are never copied.
 !
 ! usage: ref_harness <outdir> <nsteps> <dump_every> <threads> [inject.dat]
+!        ref_harness <outdir> <nsteps> 0 <threads> [warmup]   (timing mode)
 !   Sequence (mirrors source/main.f90:53-166 without its file output):
 !   init_param -> init_particle -> [inject state] -> init_virtual_markers
 !   -> init_cell -> update_cell -> ghost (the initial fill that
 !   write_param_mod.f90:144 performs) -> dumps -> nsteps x (RK, PST or
 !   check_steady_state).  dump_every = 0 selects timing mode: no dumps,
-!   jittered lattice, wall-clock of the loop printed as
-!   "particle_steps_per_s <value>".
+!   jittered lattice, `warmup` untimed steps first (default 0), then the
+!   wall-clock of nsteps steps printed as "particle_steps_per_s <value>".
 
 #include "config.h"
 
@@ -37,7 +38,7 @@ program ref_harness
     type(VM_type)    :: VM, VM2
     type(Cell_type)  :: cell, cell2
     character(len=1024) :: outdir, arg, inject
-    integer :: nsteps, dump_every, nthreads, step, is_steady, un, i
+    integer :: nsteps, dump_every, nthreads, step, is_steady, un, i, warmup
     double precision :: t0, t1, rnd, df
 
     call get_command_argument(1, outdir)
@@ -45,7 +46,15 @@ program ref_harness
     call get_command_argument(3, arg); read(arg, *) dump_every
     call get_command_argument(4, arg); read(arg, *) nthreads
     inject = ""
-    if (command_argument_count() >= 5) call get_command_argument(5, inject)
+    warmup = 0
+    if (command_argument_count() >= 5) then
+        call get_command_argument(5, arg)
+        if (dump_every == 0) then
+            read(arg, *) warmup
+        else
+            inject = arg
+        endif
+    endif
 
     call init_param(param)
     call omp_set_dynamic(.false.)
@@ -93,7 +102,8 @@ program ref_harness
     endif
 
     t0 = omp_get_wtime()
-    do step = 1, nsteps
+    do step = 1, warmup + nsteps
+        if (step == warmup + 1) t0 = omp_get_wtime()
 #if (RK == 2)
         call RK2(param, SP, VM, cell)
 #else
