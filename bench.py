@@ -70,10 +70,12 @@ class HipEvent:
         return float(ms.value)
 
 
-def run_gpu(param, st, precision: int, steps: int, warmup: int) -> dict:
+def run_gpu(param, st, precision: int, steps: int, warmup: int, setup=None) -> dict:
     print(f"[bench] precision={'mixed' if precision else 'fp64'}: upload {st.num_int} particles", flush=True)
     t_up = time.perf_counter()
     sim = simulation_from_state(param, st, precision=precision)
+    if setup:
+        setup(sim)
     sim.synchronize()
     up_ms = (time.perf_counter() - t_up) * 1e3
     sim.eager_step(warmup)
@@ -234,7 +236,9 @@ def model_bench(args) -> None:
     n = args.n
     base, st = workloads.taylor_green_periodic(n=n, re=1000.0, rk=4, jitter=0.2, seed=2029)
     param = lcfg.init_param(base.cfg.replace(sph_model=args.sph_model))
-    r = run_gpu(param, st, 0, args.steps, args.warmup)
+    ppw = args.wave_particles
+    r = run_gpu(param, st, 0, args.steps, args.warmup,
+                setup=(lambda sim: sim.set_wave_particles(ppw)) if ppw else None)
     kt = r["kt"]
     n_part = st.num_int
     print(json.dumps({
@@ -248,6 +252,7 @@ def model_bench(args) -> None:
                                   for k in ("lsa", "pst", "cells", "ghost", "other")},
         "operator_ms_per_launch": {"rhs": round(kt.lsa_ms / max(kt.lsa_launches, 1), 4),
                                    "pst": round(kt.pst_ms / max(kt.pst_launches, 1), 4)},
+        "wave_particles": ppw or "default",
         "tg_l2_error": r["l2"]}), flush=True)
 
 
@@ -412,6 +417,8 @@ def main():
                     help="only run 3 device-to-device copies of BYTES (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--sph-model", type=int, default=3,
                     help="1-2 classical SPH, 4-6 LS q=9/14/20: time that operator (FP64) instead of the headline")
+    ap.add_argument("--wave-particles", type=int, default=0,
+                    help="particles per wavefront of the SPH_MODEL 5-6 kernels (0: the library's default)")
     ap.add_argument("--workload", choices=("taylor-green", "ensemble", "boussinesq"), default="taylor-green",
                     help="ensemble: 64 lid-driven cavities (autotuned, graph-replayed); boussinesq: 1024^2 "
                          "convection, mixed vs FP64; each prints a line of its own, not the headline")

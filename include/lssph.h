@@ -299,6 +299,13 @@ int32_t lssph_set_skin(lssph_handle_t h, double skin);
    results are bitwise identical. */
 int32_t lssph_set_scan(lssph_handle_t h, int32_t staged);
 
+/* Particles per wavefront of the SPH_MODEL 6 kernels (q = 20), a tuning
+   knob with no reference counterpart: a group of 64 / ppw lanes gathers,
+   sums and factors one particle's 20 x 20 system.  1 or 2; 0 restores the
+   default (2).  Both values give the same results bit for bit; the other
+   models (one particle per lane) ignore it. */
+int32_t lssph_set_wave_particles(lssph_handle_t h, int32_t ppw);
+
 /* The HIP stream (hipStream_t) the handle launches on, for event timing. */
 void *lssph_stream(lssph_handle_t h);
 

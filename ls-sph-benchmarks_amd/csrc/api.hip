@@ -179,6 +179,9 @@ InstDev make_inst(const lssph_params_t &p, double skin) {
   I.delta = p.coe_delta_sph * p.sound_speed * p.h * p.zeta_rsst;
   I.pst_c = p.pst_c;
   I.dt = p.dt;
+  I.dt_half = p.dt / 2.0;
+  I.dt_third = p.dt / 3.0;
+  I.dt_sixth = p.dt / 6.0;
   I.u_top = p.u_top;
   I.tem_top = p.tem_top;
   I.tem_bottom = p.tem_bottom;
@@ -903,7 +906,7 @@ int32_t lssph_pair_lists(lssph_handle_t h, int32_t cap, int32_t *counts, int32_t
   if (int rc = ready(h)) return rc;
   if (!counts || !list || !path || cap < 1) return fail(h, LSSPH_E_ARG, "lssph_pair_lists: null array or cap < 1");
   if (h->d.opt.sph_model == 6)
-    return fail(h, LSSPH_E_ARG, "lssph_pair_lists: the SPH_MODEL 6 kernels gather per wavefront (lssph_neighbors)");
+    return fail(h, LSSPH_E_ARG, "lssph_pair_lists: the SPH_MODEL 6 kernels gather per lane group (lssph_neighbors)");
   Device &d = h->d;
   hipStream_t s = h->stream;
   const size_t ni = std::max(d.n_int, 1);
@@ -1081,6 +1084,14 @@ int32_t lssph_instance_status(lssph_handle_t h, int32_t *retired, int32_t *reaso
     HIP_TRY(h, hipMemcpyAsync(reason, d.retired_buf + d.n_inst, sizeof(int32_t) * d.n_inst, hipMemcpyDeviceToHost,
                               h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int32_t lssph_set_wave_particles(lssph_handle_t h, int32_t ppw) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (!(ppw == 0 || ppw == 1 || ppw == 2)) return fail(h, LSSPH_E_ARG, "lssph_set_wave_particles: 0, 1 or 2");
+  h->d.wave_ppw = ppw;
+  drop_graphs(h);
   return 0;
 }
 

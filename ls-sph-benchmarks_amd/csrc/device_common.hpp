@@ -48,6 +48,7 @@ struct InstDev {
   double rho_ref, vis_ref, c_sq, thermal_dif, alpha_ref, gravity, tem_ave;
   double zeta_sq, xi_sq, delta;  // delta = coe_delta_sph * c * h * zeta
   double pst_c, dt, u_top;
+  double dt_half, dt_third, dt_sixth;  // dt / 2, dt / 3, dt / 6 (RK stage weights, formed once)
   double tem_top, tem_bottom, tem_left, tem_right;
   double diff_coef, diff_source, cond_max;
   double x_min, x_max, y_min, y_max;  // fluid box (walls beyond)
@@ -77,13 +78,15 @@ __device__ __forceinline__ float fast_rcpf(float x) { return __builtin_amdgcn_rc
 // without the normalisation factor.
 // ---------------------------------------------------------------------------
 // KT > 0 fixes the kernel at compile time (the runtime kt is then ignored);
-// KT = 0 selects it at run time.
-template <typename T, int KT = 0>
+// KT = 0 selects it at run time.  INSIDE: the caller guarantees q <= 2 (a
+// neighbor accepted by r^2 <= h_eff^2), so the Wendland C2 support test is
+// left out (in FP32, q may round a few ulp above 2, where t^4 ~ 1e-29).
+template <typename T, int KT = 0, bool INSIDE = false>
 __device__ __forceinline__ T kshape_w(int kt_rt, T q) {
   const int kt = KT ? KT : kt_rt;
   const T one = T(1), half = T(0.5), two = T(2);
   if (kt == 3) {
-    if (q > two) return T(0);
+    if (!INSIDE && q > two) return T(0);
     T t = one - half * q, t2 = t * t;
     return (t2 * t2) * (one + two * q);
   }

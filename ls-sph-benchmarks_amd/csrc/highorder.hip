@@ -3,29 +3,64 @@
 // same basis (shift_interpolate_mod.f90:45-52, 153-185, 255-298), FP64.
 //
 // A 20 x 20 moment matrix (210 sums plus 20 per field) does not fit one
-// thread's registers, so one wavefront handles one particle:
-//   1. the lanes test the stencil's candidates 64 at a time and compact the
+// thread's registers, so a group of GS lanes (32 or 64: 2 or 1 particles per
+// wavefront, GS >= q) handles one particle:
+//   1. the group tests the stencil's candidates GS at a time and compacts the
 //      accepted ones into an LDS list with a ballot;
 //   2. a rank sort puts the list into the reference's summation order
 //      (position of the neighbor's reference cell in the 3x3 stencil of the
 //      particle's reference cell, then upload index): at q = 20, whose
 //      matrices reach cond(M) ~ 4e7, the summation order shows in the results;
-//   3. in chunks of 32 neighbors, lane k forms the basis a, VW * a and the
-//      field differences of neighbor k in LDS; then every lane accumulates
-//      its share of the sums over the chunk, in list order;
-//   4. the wave factors the symmetrized matrix in LDS (LU with partial
+//   3. in chunks of kChunk neighbors, lane k of the group forms the basis a,
+//      VW * a and the field differences of neighbor k in LDS; then every lane
+//      accumulates its share of the sums over the chunk, in list order;
+//   4. the group factors the symmetrized matrix in LDS (LU with partial
 //      pivoting, the operation order of lu_factor / lu_apply) and solves;
-//   5. lane 0 runs the epilogue of the q = 5 kernels (RHS, stage update,
-//      cell bookkeeping).
-// Waves never wait for each other (no block barrier), so a wave whose slot
-// is a wall or past the end simply returns.  (q = 9 and 14 run on the
-// per-thread kernels; the templates below also take them.)
+//   5. the group's first lane runs the epilogue of the q = 5 kernels (RHS,
+//      stage update, cell bookkeeping).
+// Each accumulator sums its neighbors in list order and the factorization's
+// operations do not depend on GS, so every group size gives the same
+// results bit for bit; GS only trades lanes against particles in flight.
+// Groups never wait for each other (no block barrier), so a group whose slot
+// is a wall or past the end simply returns.  (The templates also take
+// q = 14, but SPH_MODEL 5 runs 4x faster on the per-thread kernels of
+// operators.hip, even with part of its sums in scratch: 7.9 against 34 ms per
+// 1024^2 step.)
 #include "ls_common.hpp"
 
 namespace lssph {
 
-constexpr int kChunk = 32;   // neighbors staged in LDS per round
-constexpr int kWavesPB = 2;  // wavefronts (particles) per block
+constexpr int kChunk = 16;   // neighbors staged in LDS per round
+constexpr int kWavesPB = 2;  // wavefronts per block
+
+// Lanes of one particle's group: GS consecutive lanes of a wavefront.
+template <int GS>
+struct Group {
+  static constexpr int G = 64 / GS;  // particles per wavefront
+  int lane;  // lane in the wavefront
+  int g;     // group in the wavefront
+  int gl;    // lane in the group
+  unsigned long long mask;  // the group's lanes
+  __device__ __forceinline__ explicit Group(int l) : lane(l), g(l / GS), gl(l % GS) {
+    mask = (GS == 64 ? ~0ull : ((1ull << GS) - 1ull)) << (g * GS);
+  }
+  // maximum over the group's lanes of (v, index), ties to the lower index
+  __device__ __forceinline__ void argmax(double &v, int &idx) const {
+#pragma unroll
+    for (int off = 1; off < GS; off <<= 1) {
+      const double v2 = __shfl_xor(v, off, GS);
+      const int i2 = __shfl_xor(idx, off, GS);
+      const bool take = (v2 > v) || (v2 == v && i2 < idx);
+      v = take ? v2 : v;
+      idx = take ? i2 : idx;
+    }
+  }
+  // the value of the group's lane k
+  template <typename T>
+  __device__ __forceinline__ T from(T v, int k) const {
+    return __shfl(v, g * GS + k, 64);
+  }
+};
 
 
 // Kernel W(r) and dW/dr written term by term as kernel_functions_mod.f90
@@ -134,15 +169,16 @@ __device__ __forceinline__ void basis_hq(double x, double y, const double (&ih)[
   }
 }
 
-// LDS of one wavefront.  Row k of `rows` holds neighbor k of the chunk:
-// a[0..Q), VW * a[0..Q), the field differences [0..NF) and the two shift
-// terms of the shifting kernel.  The factorization reuses the same space.
+// LDS of one particle's group.  Row k of `rows` holds neighbor k of the
+// chunk: a[0..Q), VW * a[0..Q), the field differences [0..NF) and the two
+// shift terms of the shifting kernel.  The factorization reuses the same
+// space.  About 7.5 KB for q = 20.
 template <int Q, int NF>
 struct WaveLds {
   static constexpr int RS = 2 * Q + NF + 2;
   int32_t list[kWaveList];    // accepted candidates, stencil order
   int32_t sorted[kWaveList];  // the same in summation order
-  long long key[kWaveList];
+  uint32_t key[kWaveList];    // stencil position << 27 | upload index (< 2^27)
   union {
     double rows[kChunk][RS];
     struct {
@@ -155,17 +191,17 @@ struct WaveLds {
 // Neighbors of the particle at slot s (record me) within h_eff, in the
 // reference's summation order, into W.sorted.  Returns the count; a count
 // above kWaveList is flagged and only the first kWaveList are kept.
-template <class WL>
-__device__ int wave_neighbors(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
-                              const int32_t *__restrict__ sinfo, const int32_t *__restrict__ items,
-                              const int32_t *__restrict__ orig_of, int32_t *counters, const InstDev &I, int s,
-                              const Rec &me, WL &W, int lane) {
+template <int GS, class WL>
+__device__ int group_neighbors(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
+                               const int32_t *__restrict__ sinfo, const int32_t *__restrict__ items,
+                               const int32_t *__restrict__ orig_of, int32_t *counters, const InstDev &I, int s,
+                               const Rec &me, WL &W, const Group<GS> &G) {
   const int lc = local_cell(I, me.x, me.y);
-  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long below = G.mask & ((1ull << G.lane) - 1ull);
   int cnt = 0;
   visit_cells(I, lc, cstart, [&](int t0, int t1, int, int) {
-    for (int base = t0; base < t1; base += 64) {
-      const int t = base + lane;
+    for (int base = t0; base < t1; base += GS) {
+      const int t = base + G.gl;
       bool ok = false;
       int code = 0;
       if (t < t1 && t != s) {
@@ -175,20 +211,20 @@ __device__ int wave_neighbors(const Rec *__restrict__ mir, const int32_t *__rest
         ok = sq_dist((p.x + sx) - me.x, (p.y + sy) - me.y) <= I.h_eff_sq;
         code = image_code(sx, sy);
       }
-      const unsigned long long m = __ballot(ok);
+      const unsigned long long m = __ballot(ok) & G.mask;
       const int pos = cnt + __popcll(m & below);
       if (ok && pos < kWaveList) W.list[pos] = t | (code << kSlotBits);
       cnt += __popcll(m);
     }
   });
   if (cnt > kWaveList) {
-    if (lane == 0) atomicAdd(&counters[kWaveListOverflow], 1);
+    if (G.gl == 0) atomicAdd(&counters[kWaveListOverflow], 1);
     cnt = kWaveList;
   }
   // sort keys: stencil position of the reference cells, then upload index
   int rcx, rcy;
   ref_cell_xy(I, me.x, me.y, false, rcx, rcy);
-  for (int e = lane; e < cnt; e += 64) {
+  for (int e = G.gl; e < cnt; e += GS) {
     const int t = W.list[e] & kSlotMask;
     const double2 p = load_xy(mir, t);
     int jx, jy;
@@ -198,12 +234,12 @@ __device__ int wave_neighbors(const Rec *__restrict__ mir, const int32_t *__rest
       dx = dx > 1 ? dx - I.rnx : (dx < -1 ? dx + I.rnx : dx);
       dy = dy > 1 ? dy - I.rny : (dy < -1 ? dy + I.rny : dy);
     }
-    W.key[e] = ((long long)((dy + 1) * 3 + (dx + 1)) << 32) | (long long)orig_of[items[t]];
+    W.key[e] = ((uint32_t)((dy + 1) * 3 + (dx + 1)) << kSlotBits) | (uint32_t)orig_of[items[t]];
   }
   wave_sync();
   // rank sort (the keys are distinct)
-  for (int e = lane; e < cnt; e += 64) {
-    const long long k = W.key[e];
+  for (int e = G.gl; e < cnt; e += GS) {
+    const uint32_t k = W.key[e];
     int rank = 0;
     for (int j = 0; j < cnt; ++j) rank += (W.key[j] < k) ? 1 : 0;
     W.sorted[rank] = W.list[e];
@@ -214,20 +250,20 @@ __device__ int wave_neighbors(const Rec *__restrict__ mir, const int32_t *__rest
 
 // Per-lane share of the LS sums: entry e < Q(Q+1)/2 is M(r, i), r <= i,
 // accumulated as (VW a_i) * a_r (LSSPH_A_mod.f90:185-188); the others are
-// b(i, f) += (VW a_i) * df_f (:178-183).  o1/o2 are the row offsets of the
-// two factors.
-template <int Q, int NF>
+// b(i, f) += (VW a_i) * df_f (:178-183).  Lane gl of the group owns entries
+// gl, gl + GS, ...; o1/o2 are the row offsets of the two factors.
+template <int Q, int NF, int GS>
 struct LaneSums {
   static constexpr int NM = Q * (Q + 1) / 2;
   static constexpr int NE = NM + Q * NF;
-  static constexpr int PER = (NE + 63) / 64;
+  static constexpr int PER = (NE + GS - 1) / GS;
   double acc[PER];
   int o1[PER], o2[PER];
-  __device__ __forceinline__ void init(int lane) {
+  __device__ __forceinline__ void init(int gl) {
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
       acc[m] = 0.0;
-      const int e = lane + 64 * m;
+      const int e = gl + GS * m;
       int i = 0, r = 0, f = -1;
       if (e < NM) {
         while ((i + 1) * (i + 2) / 2 <= e) ++i;
@@ -241,20 +277,20 @@ struct LaneSums {
     }
   }
   template <class WL>
-  __device__ __forceinline__ void add_chunk(const WL &W, int nc, int lane) {
+  __device__ __forceinline__ void add_chunk(const WL &W, int nc, int gl) {
     for (int k = 0; k < nc; ++k) {
       const double *row = W.u.rows[k];
 #pragma unroll
       for (int m = 0; m < PER; ++m)
-        if (lane + 64 * m < NE) acc[m] += row[o1[m]] * row[o2[m]];
+        if (gl + GS * m < NE) acc[m] += row[o1[m]] * row[o2[m]];
     }
   }
   // symmetrized matrix and right-hand sides into W.u.s (after a wave_sync)
   template <class WL>
-  __device__ __forceinline__ void store(WL &W, int lane) const {
+  __device__ __forceinline__ void store(WL &W, int gl) const {
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
-      const int e = lane + 64 * m;
+      const int e = gl + GS * m;
       if (e < NM) {
         const int i = o1[m] - Q, r = o2[m];
         W.u.s.m[r][i] = acc[m];
@@ -266,46 +302,41 @@ struct LaneSums {
   }
 };
 
-// DGESV on W.u.s by one wavefront: LU with partial pivoting in the order
-// of lu_factor (first row of the largest |a| as pivot; a zero pivot sets
-// info and skips the scaling), then -- when no pivot is zero -- the row
+// DGESV on W.u.s by one group: LU with partial pivoting in the order of
+// lu_factor (first row of the largest |a| as pivot; a zero pivot sets info
+// and skips the scaling), then -- when no pivot is zero -- the row
 // interchanges, forward and back substitution of lu_apply.  Returns info.
-template <int Q, int NF, class WL>
-__device__ int wave_dgesv(WL &W, int lane) {
+template <int Q, int NF, int GS, class WL>
+__device__ int group_dgesv(WL &W, const Group<GS> &G) {
+  static_assert(GS >= Q, "a group holds one row of the matrix per lane");
   auto &m = W.u.s.m;
   auto &b = W.u.s.b;
+  const int gl = G.gl;
   int piv[Q];
   int info = 0;
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
-    double v = (lane >= j && lane < Q) ? fabs(m[lane < Q ? lane : 0][j]) : -1.0;
-    int idx = lane;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const double v2 = __shfl_xor(v, off);
-      const int i2 = __shfl_xor(idx, off);
-      const bool take = (v2 > v) || (v2 == v && i2 < idx);
-      v = take ? v2 : v;
-      idx = take ? i2 : idx;
-    }
-    int p = __builtin_amdgcn_readfirstlane(idx);
+    double v = (gl >= j && gl < Q) ? fabs(m[gl < Q ? gl : 0][j]) : -1.0;
+    int idx = gl;
+    G.argmax(v, idx);
+    int p = G.from(idx, 0);
     p = (p >= j && p < Q) ? p : j;  // only a column of NaNs leaves no pivot row
     piv[j] = p;
-    if (p != j && lane < Q) {
-      const double t = m[j][lane];
-      m[j][lane] = m[p][lane];
-      m[p][lane] = t;
+    if (p != j && gl < Q) {
+      const double t = m[j][gl];
+      m[j][gl] = m[p][gl];
+      m[p][gl] = t;
     }
     wave_sync();
     const double pv = m[j][j];
     if (pv == 0.0) {
       info = info ? info : j + 1;
-    } else if (lane > j && lane < Q) {
-      m[lane][j] = m[lane][j] / pv;
+    } else if (gl > j && gl < Q) {
+      m[gl][j] = m[gl][j] / pv;
     }
     wave_sync();
     const int nn = Q - 1 - j;
-    for (int e = lane; e < nn * nn; e += 64) {
+    for (int e = gl; e < nn * nn; e += GS) {
       const int i = j + 1 + e / nn, c = j + 1 + e % nn;
       m[i][c] -= m[i][j] * m[j][c];
     }
@@ -315,17 +346,17 @@ __device__ int wave_dgesv(WL &W, int lane) {
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
     const int p = piv[j];
-    if (p != j && lane < NF) {
-      const double t = b[j][lane];
-      b[j][lane] = b[p][lane];
-      b[p][lane] = t;
+    if (p != j && gl < NF) {
+      const double t = b[j][gl];
+      b[j][gl] = b[p][gl];
+      b[p][gl] = t;
     }
   }
   wave_sync();
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
     const int rows = (Q - 1 - j) * NF;
-    for (int e = lane; e < rows; e += 64) {
+    for (int e = gl; e < rows; e += GS) {
       const int i = j + 1 + e / NF, f = e % NF;
       b[i][f] -= b[j][f] * m[i][j];
     }
@@ -333,9 +364,9 @@ __device__ int wave_dgesv(WL &W, int lane) {
   }
 #pragma unroll
   for (int j = Q - 1; j >= 0; --j) {
-    if (lane < NF) b[j][lane] = b[j][lane] / m[j][j];
+    if (gl < NF) b[j][gl] = b[j][gl] / m[j][j];
     wave_sync();
-    for (int e = lane; e < j * NF; e += 64) {
+    for (int e = gl; e < j * NF; e += GS) {
       const int i = e / NF, f = e % NF;
       b[i][f] -= b[j][f] * m[i][j];
     }
@@ -352,38 +383,40 @@ __device__ __forceinline__ void inv_h_powers(const InstDev &I, double (&ih)[5]) 
 }
 
 // ---------------------------------------------------------------------------
-// LS type A (LSSPH_A_mod.f90:26-271) with q = 9, 14, 20 and the stage update.
+// LS type A (LSSPH_A_mod.f90:26-271) with q = 14, 20 and the stage update.
 // The field columns are those of the q = 5 kernels with rho always carried.
 // ---------------------------------------------------------------------------
-template <int Q, int CLS>
+template <int Q, int CLS, int GS>
 __global__ void __launch_bounds__(64 * kWavesPB) k_lsa_wave(LsaArgs A) {
   using L = Layout<CLS, true>;
   constexpr int NF = L::NF;
   using WL = WaveLds<Q, NF>;
-  __shared__ WL lds_all[kWavesPB];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int s = blockIdx.x * kWavesPB + w;
+  constexpr int NG = 64 / GS;
+  __shared__ WL lds_all[kWavesPB * NG];
+  const Group<GS> G(threadIdx.x & 63);
+  const int w = threadIdx.x >> 6;
+  const int s = (blockIdx.x * kWavesPB + w) * NG + G.g;
   if (s >= A.n) return;
-  WL &W = lds_all[w];
+  WL &W = lds_all[w * NG + G.g];
   const int32_t inf = A.sinfo[s];
   const InstDev &I = A.inst[info_inst(inf)];
   // walls, and the particles of retired instances, move on with the records
   if (info_ptype(inf) != 0 || I.retired) {
-    if (lane == 0 && A.mode != kRhsOnly) A.mout[s] = A.mir[s];
-    if (lane == 0 && A.spd) A.spd[s] = 0.0;
+    if (G.gl == 0 && A.mode != kRhsOnly) A.mout[s] = A.mir[s];
+    if (G.gl == 0 && A.spd) A.spd[s] = 0.0;
     return;
   }
   const Rec me = A.mir[s];
-  const int n = wave_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.counters, I, s, me, W, lane);
+  const int n = group_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.counters, I, s, me, W, G);
   double ih[5];
   inv_h_powers(I, ih);
   const int kt = A.kt;
-  LaneSums<Q, NF> S;
-  S.init(lane);
+  LaneSums<Q, NF, GS> S;
+  S.init(G.gl);
   for (int c0 = 0; c0 < n; c0 += kChunk) {
     const int nc = min(kChunk, n - c0);
-    if (lane < nc) {
-      const int e = W.sorted[c0 + lane];
+    if (G.gl < nc) {
+      const int e = W.sorted[c0 + G.gl];
       const int t = e & kSlotMask, code = e >> kSlotBits;
       const Rec nb = A.mir[t];
       const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
@@ -394,7 +427,7 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_lsa_wave(LsaArgs A) {
       basis_hq<Q>(dx, dy, ih, a);
       double df[NF];
       field_diffs<CLS, true>(nb, me, df);
-      double *row = W.u.rows[lane];
+      double *row = W.u.rows[G.gl];
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         row[i] = a[i];
@@ -404,14 +437,14 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_lsa_wave(LsaArgs A) {
       for (int f = 0; f < NF; ++f) row[2 * Q + f] = df[f];
     }
     wave_sync();
-    S.add_chunk(W, nc, lane);
+    S.add_chunk(W, nc, G.gl);
     wave_sync();
   }
-  S.store(W, lane);
+  S.store(W, G.gl);
   wave_sync();
   double d[NF][5];
   if (n >= Q) {  // LSSPH_A_mod.f90:208 (num_neighbors >= q)
-    wave_dgesv<Q, NF>(W, lane);
+    group_dgesv<Q, NF>(W, G);
     double X[NF][5];
 #pragma unroll
     for (int f = 0; f < NF; ++f)
@@ -424,40 +457,42 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_lsa_wave(LsaArgs A) {
 #pragma unroll
       for (int i = 0; i < 5; ++i) d[f][i] = 0.0;
   }
-  if (lane == 0) finish<CLS, true>(A, I, A.items[s], s, me, d);
+  if (G.gl == 0) finish<CLS, true>(A, I, A.items[s], s, me, d);
 }
 
 // ---------------------------------------------------------------------------
-// Particle shifting (shift_interpolate_mod.f90:23-300) with q = 9, 14, 20.
+// Particle shifting (shift_interpolate_mod.f90:23-300) with q = 14, 20.
 // ---------------------------------------------------------------------------
-template <int Q, bool T>
+template <int Q, bool T, int GS>
 __global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
   constexpr int NF = T ? 4 : 3;  // u, v, rho (, tem)
   using WL = WaveLds<Q, NF>;
-  __shared__ WL lds_all[kWavesPB];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int s = blockIdx.x * kWavesPB + w;
+  constexpr int NG = 64 / GS;
+  __shared__ WL lds_all[kWavesPB * NG];
+  const Group<GS> G(threadIdx.x & 63);
+  const int w = threadIdx.x >> 6;
+  const int s = (blockIdx.x * kWavesPB + w) * NG + G.g;
   if (s >= A.n) return;
-  WL &W = lds_all[w];
+  WL &W = lds_all[w * NG + G.g];
   const int32_t inf = A.sinfo[s];
   const int ii = info_inst(inf);
   const InstDev &I = A.inst[ii];
   if (info_ptype(inf) != 0 || I.retired) {
-    if (lane == 0) A.mout[s] = A.mir[s];  // walls move on with the records
+    if (G.gl == 0) A.mout[s] = A.mir[s];  // walls move on with the records
     return;
   }
   const Rec me = A.mir[s];
-  const int n = wave_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.mv.counters, I, s, me, W, lane);
+  const int n = group_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.mv.counters, I, s, me, W, G);
   double ih[5];
   inv_h_powers(I, ih);
   const int kt = A.kt;
-  LaneSums<Q, NF> S;
-  S.init(lane);
+  LaneSums<Q, NF, GS> S;
+  S.init(G.gl);
   double sx = 0.0, sy = 0.0;
   for (int c0 = 0; c0 < n; c0 += kChunk) {
     const int nc = min(kChunk, n - c0);
-    if (lane < nc) {
-      const int e = W.sorted[c0 + lane];
+    if (G.gl < nc) {
+      const int e = W.sorted[c0 + G.gl];
       const int t = e & kSlotMask, code = e >> kSlotBits;
       const Rec nb = A.mir[t];
       const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
@@ -473,7 +508,7 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
       const double fac = 1.0 + 0.2 * ((wr * wr) * (wr * wr));
       double a[Q];
       basis_hq<Q>(dx, dy, ih, a);
-      double *row = W.u.rows[lane];
+      double *row = W.u.rows[G.gl];
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         row[i] = a[i];
@@ -491,10 +526,10 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
       sx = sx + W.u.rows[k][2 * Q + NF];
       sy = sy + W.u.rows[k][2 * Q + NF + 1];
     }
-    S.add_chunk(W, nc, lane);
+    S.add_chunk(W, nc, G.gl);
     wave_sync();
   }
-  S.store(W, lane);
+  S.store(W, G.gl);
   wave_sync();
   // shift vector and its clamp (:217-225)
   const double umax = __longlong_as_double((long long)A.umax[ii]);
@@ -507,10 +542,10 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
   }
   bool solved = false;
   if (n >= Q) {  // :240-247
-    wave_dgesv<Q, NF>(W, lane);
+    group_dgesv<Q, NF>(W, G);
     solved = true;
   }
-  if (lane != 0) return;
+  if (G.gl != 0) return;
   double a[Q];
   basis_hq<Q>(dxs, dys, ih, a);
   double acc[NF];
@@ -538,37 +573,45 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// host launchers
+// host launchers: groups of 64 / ppw lanes (ppw particles per wavefront)
 // ---------------------------------------------------------------------------
-template <int Q>
-static void lsa_wave_q(hipStream_t s, const LsaArgs &a, int cls) {
-  const dim3 g((a.n + kWavesPB - 1) / kWavesPB), b(64 * kWavesPB);
+template <int Q, int GS>
+static void lsa_wave_g(hipStream_t s, const LsaArgs &a, int cls) {
+  constexpr int per_block = kWavesPB * (64 / GS);
+  const dim3 g((a.n + per_block - 1) / per_block), b(64 * kWavesPB);
   if (cls == 0)
-    hipLaunchKernelGGL((k_lsa_wave<Q, 0>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_lsa_wave<Q, 0, GS>), g, b, 0, s, a);
   else if (cls == 1)
-    hipLaunchKernelGGL((k_lsa_wave<Q, 1>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_lsa_wave<Q, 1, GS>), g, b, 0, s, a);
   else
-    hipLaunchKernelGGL((k_lsa_wave<Q, 2>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_lsa_wave<Q, 2, GS>), g, b, 0, s, a);
 }
 
-void launch_lsa_wave(hipStream_t s, Device &, const LsaArgs &a, int, int cls) {
+void launch_lsa_wave(hipStream_t s, Device &d, const LsaArgs &a, int, int cls) {
   if (a.n == 0) return;
-  lsa_wave_q<20>(s, a, cls);
+  if (wave_particles(d) == 2)
+    lsa_wave_g<20, 32>(s, a, cls);
+  else
+    lsa_wave_g<20, 64>(s, a, cls);
 }
 
-template <int Q>
-static void pst_wave_q(hipStream_t s, const PstArgs &a, bool tem) {
-  const dim3 g((a.n + kWavesPB - 1) / kWavesPB), b(64 * kWavesPB);
+template <int Q, int GS>
+static void pst_wave_g(hipStream_t s, const PstArgs &a, bool tem) {
+  constexpr int per_block = kWavesPB * (64 / GS);
+  const dim3 g((a.n + per_block - 1) / per_block), b(64 * kWavesPB);
   if (tem)
-    hipLaunchKernelGGL((k_pst_wave<Q, true>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_pst_wave<Q, true, GS>), g, b, 0, s, a);
   else
-    hipLaunchKernelGGL((k_pst_wave<Q, false>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_pst_wave<Q, false, GS>), g, b, 0, s, a);
 }
 
-void launch_pst_wave(hipStream_t s, Device &d, const PstArgs &a, int sph_model) {
+void launch_pst_wave(hipStream_t s, Device &d, const PstArgs &a, int) {
   if (a.n == 0) return;
-  (void)sph_model;
-  pst_wave_q<20>(s, a, d.opt.has_tem != 0);
+  const bool tem = d.opt.has_tem != 0;
+  if (wave_particles(d) == 2)
+    pst_wave_g<20, 32>(s, a, tem);
+  else
+    pst_wave_g<20, 64>(s, a, tem);
 }
 
 }  // namespace lssph

@@ -36,9 +36,9 @@ constexpr long long kMaxParticles = kSlotMask;
 // more than kStrayCap of them in one stage is reported as an error.
 constexpr int kStrayCap = 4096;
 constexpr int kArrivals = 8;  // arrivals per new cell merged directly
-// Neighbor-list capacity per particle of the one-wavefront-per-particle
-// kernels (SPH_MODEL 6).
-constexpr int kWaveList = 256;
+// Neighbor-list capacity per particle of the lane-group kernels (SPH_MODEL
+// 6).
+constexpr int kWaveList = 128;
 enum { kFallbacks = 0, kOverflows = 1, kNumStats64 = 2 };
 
 // Raw device pointers and sizes; passed to the launchers.
@@ -98,8 +98,13 @@ struct Device {
   int block = 128;
   int staged_scan = 1;  // lssph_set_scan
   int list_len = 39;     // LDS list entries per thread, the particle itself included
+  int wave_ppw = 0;      // particles per wavefront of the SPH_MODEL 6 kernels (0: default)
   int red_blocks = 256;  // blocks per instance of the max reductions
 };
+
+// Particles per wavefront of the lane-group kernels of SPH_MODEL 6 (1 or
+// 2); the default is the faster setting measured at 1024^2.
+inline int wave_particles(const Device &d) { return d.wave_ppw ? d.wave_ppw : 2; }
 
 // cells.hip
 void launch_fill_i32(hipStream_t s, int32_t *p, int32_t v, int n);

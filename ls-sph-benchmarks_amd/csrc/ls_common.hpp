@@ -197,6 +197,17 @@ __device__ __forceinline__ double sym5(const R (&M)[15], int i, int c) {
 // on one-sided supports it falls short of kappa_1 by up to 12x.)
 __device__ __forceinline__ double basis5_weight(int i) { return (i == 2 || i == 3) ? 0.5 : 1.0; }
 
+// 1 / d for a positive normal d: the hardware reciprocal and two Newton
+// steps (within an ulp; the correctly rounded division costs about twice as
+// many instructions).  Used by the mixed path's factorization only.
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
 template <int NF>
 __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, double (&X)[NF][5],
                                                   double cond_max) {
@@ -209,7 +220,7 @@ __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, do
     for (int k = 0; k < j; ++k) dj -= L[j][k] * L[j][k] * d[k];
     ok = ok && (dj > 0.0);
     d[j] = dj;
-    dinv[j] = 1.0 / dj;
+    dinv[j] = recip(dj);
 #pragma unroll
     for (int i = j + 1; i < 5; ++i) {
       double t = sym5(S.M, i, j);
@@ -374,7 +385,7 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
                    const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
                    const float q = fast_sqrtf(xs * xs + ys * ys);
                    // the common factor w_norm of M and b cancels in the solve
-                   const float vw = float(nb.vol) * kshape_w<float, KT>(kt, q);
+                   const float vw = float(nb.vol) * kshape_w<float, KT, true>(kt, q);
                    float a[5];
                    basis5s(xs, ys, a);
                    float df[NF];
@@ -430,7 +441,7 @@ __device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p
   const double dt = I.dt;
   Rec out = me;
   if (mode == kRk2S1 || mode == kRk2S2) {
-    const double dte = (mode == kRk2S1) ? dt / 2.0 : dt;
+    const double dte = (mode == kRk2S1) ? I.dt_half : dt;
     double ox, oy, ou, ov, orho, otem;
     if (mode == kRk2S1) {
       ox = me.x; oy = me.y; ou = me.u; ov = me.v; orho = me.rho; otem = me.tem;
@@ -457,8 +468,8 @@ __device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p
     if (T) out.tem = otem + r_tem * dte;
   } else {
     const int st = mode - kRk4S1 + 1;
-    const double dt1 = (st == 1 || st == 4) ? dt / 6.0 : dt / 3.0;
-    const double dt2 = (st <= 2) ? dt / 2.0 : (st == 3 ? dt : 0.0);
+    const double dt1 = (st == 1 || st == 4) ? I.dt_sixth : I.dt_third;
+    const double dt2 = (st <= 2) ? I.dt_half : (st == 3 ? dt : 0.0);
     double nx_, ny_, nu_, nv_, nrho_, ntem_;
     if (st == 1) {
       A.old_[2][p] = me.u;

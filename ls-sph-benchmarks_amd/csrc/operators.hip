@@ -26,8 +26,8 @@
 namespace lssph {
 
 // Q = 5 (SPH_MODEL 3), 9 or 14 (SPH_MODEL 4-5, FP64; at q = 14 part of the
-// sums live in scratch, still several times faster than one wavefront per
-// particle).
+// sums live in scratch, and it is still 4x faster than a group of lanes per
+// particle).  q = 20 runs on the lane-group kernels of highorder.hip.
 template <int CLS, bool MIXED, bool DELTA, int KT, int Q>
 __device__ __forceinline__ void lsa_particle(const LsaArgs &A, const InstDev &I, int s, int tid, int32_t *lds) {
   constexpr int NF = Layout<CLS, DELTA>::NF;
@@ -270,7 +270,7 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
                        // (-5 t^3) (-xs): no division by q, and the particle
                        // itself (xs = ys = 0) adds nothing
                        const float t = 1.0f - 0.5f * q, t3 = t * t * t;
-                       shp = q > 2.0f ? 0.0f : t3 * t * (1.0f + 2.0f * q);
+                       shp = t3 * t * (1.0f + 2.0f * q);  // q <= 2: an accepted neighbor
                        g = vol * float(I.w_norm * I.dw_fac) * (-5.0f * t3);
                      } else {
                        shp = kshape_w<float, KT>(kt, q);
@@ -598,7 +598,8 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
       hipLaunchKernelGGL((k_lsa<2, false, false, 0, 9>), g, b, lds, s, a);
     return;
   }
-  if (model == 5) {  // q = 14, FP64, one thread per particle (part of the sums in scratch)
+  if (model == 5) {  // q = 14, FP64, one thread per particle (part of the sums in scratch;
+                     // still faster than a lane group per particle)
     const dim3 g(xcd_grid(a.n, d.block)), b(d.block);
     if (cls == 0)
       hipLaunchKernelGGL((k_lsa<0, false, false, 0, 14>), g, b, lds, s, a);
@@ -612,7 +613,7 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
       hipLaunchKernelGGL((k_lsa<2, false, false, 0, 14>), g, b, lds, s, a);
     return;
   }
-  if (model == 6) {  // q = 20: one wavefront per particle, FP64
+  if (model == 6) {  // q = 20: a group of lanes per particle, FP64
     launch_lsa_wave(s, d, a, model, cls);
     return;
   }
@@ -719,7 +720,7 @@ void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
       hipLaunchKernelGGL((k_pst<false, false, 0, 14>), g, b, lds, s, a);
     return;
   }
-  if (model == 6) {  // q = 20: one wavefront per particle
+  if (model == 6) {  // q = 20: a group of lanes per particle
     launch_pst_wave(s, d, a, model);
     return;
   }

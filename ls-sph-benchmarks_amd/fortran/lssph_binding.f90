@@ -193,6 +193,11 @@ module lssph_binding
             real(c_double), intent(in) :: x(*), y(*)
             real(c_double), intent(out) :: out(*)
         end function
+        integer(c_int32_t) function lssph_set_wave_particles(h, ppw) bind(C, name="lssph_set_wave_particles")
+            import :: c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: ppw
+        end function
         integer(c_int32_t) function lssph_set_isolation(h, on) bind(C, name="lssph_set_isolation")
             import :: c_int32_t, c_ptr
             type(c_ptr), value :: h

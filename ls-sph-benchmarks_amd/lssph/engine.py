@@ -53,7 +53,7 @@ EXPORTS = ("lssph_create", "lssph_destroy", "lssph_last_error", "lssph_upload", 
            "lssph_neighbors", "lssph_stats", "lssph_reset_stats", "lssph_synchronize",
            "lssph_set_tuning", "lssph_set_skin", "lssph_set_scan", "lssph_stream", "lssph_profile", "lssph_kernel_times",
            "lssph_pair_lists", "lssph_probe", "lssph_set_isolation", "lssph_instance_status",
-           "lssph_probe_static")
+           "lssph_probe_static", "lssph_set_wave_particles")
 
 
 def load_library(path: Path | None = None):
@@ -95,6 +95,7 @@ def load_library(path: Path | None = None):
     lib.lssph_probe.argtypes = [vp, i32, i32, i32, i32, _D, _D, _D]
     lib.lssph_probe_static.argtypes = [vp, i32, i32, i32, i32, _D, _D, _D]
     lib.lssph_set_isolation.argtypes = [vp, i32]
+    lib.lssph_set_wave_particles.argtypes = [vp, i32]
     lib.lssph_instance_status.argtypes = [vp, _I, _I]
     for name in EXPORTS:
         if name not in ("lssph_last_error", "lssph_stream"):
@@ -335,6 +336,10 @@ class Simulation:
     def set_skin(self, skin: float):
         """Cell-list skin as a fraction of h_eff (0: rebuild after every pass)."""
         self._check(self.lib.lssph_set_skin(self.h, ctypes.c_double(skin)))
+
+    def set_wave_particles(self, ppw: int):
+        """Particles per wavefront of the SPH_MODEL 6 kernels (1, 2; 0: default)."""
+        self._check(self.lib.lssph_set_wave_particles(self.h, int(ppw)))
 
     def set_scan(self, staged: bool):
         """Wave-staged candidate scan on (default) or off; results are bitwise identical."""

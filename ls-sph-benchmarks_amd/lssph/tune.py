@@ -1,8 +1,14 @@
 """Autotuner of the pair kernels' launch settings for one workload.
 
-The settings (lssph_set_tuning, lssph_set_skin) change which candidates are
-tested and how the work is laid out on the GPU, never the neighbor sets, so
-the results of every candidate setting are the same; only the time differs.
+The settings change how the work is laid out on the GPU, never the neighbor
+sets:
+  threads per block and on-chip list length (lssph_set_tuning), particles
+      per wavefront of the SPH_MODEL 6 kernels (lssph_set_wave_particles):
+      the results are the same bit for bit;
+  cell-list skin (lssph_set_skin): cells of another width put the neighbors
+      of a stencil row in another order, so the sums round differently (the
+      FP64 path's trajectories stay within 1e-12 of each other, the mixed
+      path's within the FP32 rounding of its sums, ~1e-10 over a few steps).
 `autotune` times a few graph-replayed steps per candidate on the caller's own
 workload and returns the fastest.  No reference counterpart: the reference
 has no launch settings.
@@ -22,6 +28,7 @@ class Setting:
     block: int        # threads per block of the pair kernels
     list_len: int     # on-chip neighbor-list entries (0: the size lssph_create chose)
     skin: float       # cell-list skin as a fraction of h_eff
+    wave_particles: int = 0  # particles per wavefront, SPH_MODEL 6 (0: the library's default)
     ms_per_step: float = float("nan")
     error: str = ""
 
@@ -36,6 +43,8 @@ class TuneResult:
         def d(s: Setting) -> dict:
             out = {"block": s.block, "list_len": s.list_len or "auto", "skin": s.skin,
                    "ms_per_step": round(s.ms_per_step, 4)}
+            if s.wave_particles:
+                out["wave_particles"] = s.wave_particles
             if s.error:
                 out["error"] = s.error
             return out
@@ -53,14 +62,14 @@ def _time_steps(sim: Simulation, steps: int) -> float:
 
 def autotune(make_sim: Callable[[float], Simulation], steps: int = 5,
              blocks: Sequence[int] = (64, 128, 256), list_lens: Sequence[int] = (0,),
-             skins: Sequence[float] = (0.04, 0.08, 0.16),
-             default: tuple = (128, 0, 0.08)) -> TuneResult:
+             skins: Sequence[float] = (0.04, 0.08, 0.16), wave_particles: Sequence[int] = (0,),
+             default: tuple = (128, 0, 0.08, 0)) -> TuneResult:
     """make_sim(skin) returns a fresh Simulation with that skin, uploaded and
     ready to step (the skin is fixed before the upload).  Every combination
-    of blocks x list_lens x skins is timed over `steps` graph-replayed steps
-    after one warm-up step; a combination the library rejects (an LDS limit,
-    or a skin too narrow for the motion per step) is recorded with its error
-    and skipped."""
+    of blocks x list_lens x skins x wave_particles is timed over `steps`
+    graph-replayed steps after one warm-up step; a combination the library
+    rejects (an LDS limit, or a skin too narrow for the motion per step) is
+    recorded with its error and skipped."""
     table = []
     # list length 0 (the size lssph_create chose) must come first on each
     # fresh handle: lssph_set_tuning's 0 keeps the current length
@@ -68,11 +77,13 @@ def autotune(make_sim: Callable[[float], Simulation], steps: int = 5,
     for skin in skins:
         sim = make_sim(skin)
         try:
-            for list_len, block in itertools.product(lens, blocks):
-                s = Setting(block, list_len, skin)
+            for list_len, block, ppw in itertools.product(lens, blocks, wave_particles):
+                s = Setting(block, list_len, skin, ppw)
                 table.append(s)
                 try:
                     sim.set_tuning(block=block, list_len=list_len)
+                    if ppw:
+                        sim.set_wave_particles(ppw)
                 except LssphError as e:  # e.g. list_len x block beyond the LDS of a block
                     s.error = str(e)
                     continue
@@ -87,5 +98,5 @@ def autotune(make_sim: Callable[[float], Simulation], steps: int = 5,
     if not ok:
         raise RuntimeError("autotune: no candidate setting ran")
     best = min(ok, key=lambda s: s.ms_per_step)
-    dflt = next((s for s in ok if (s.block, s.list_len, s.skin) == tuple(default)), best)
+    dflt = next((s for s in ok if (s.block, s.list_len, s.skin, s.wave_particles) == tuple(default)), best)
     return TuneResult(best, dflt, table)

@@ -625,9 +625,9 @@ def test_results_do_not_depend_on_the_skin(skin):
 @pytest.mark.parametrize("model", [1, 2, 4, 5, 6])
 def test_periodic_other_sph_models_match_oracle(model):
     """Classical SPH (SPH_MODEL 1/2, with the q = 2 shifting) and LS-SPH with
-    the 3rd- to 5th-order Taylor bases (q = 9, 14, 20; one wavefront per
-    particle) in the periodic box: neighbor lists, the first RHS and three
-    RK4 + PST steps against the oracle."""
+    the 3rd- to 5th-order Taylor bases (q = 9, 14 per thread; q = 20 a lane
+    group per particle) in the periodic box: neighbor lists, the first RHS
+    and three RK4 + PST steps against the oracle."""
     from lssph import config as lcfg
     base, st = workloads.taylor_green_periodic(n=32, re=100.0, rk=4, jitter=0.2, seed=40 + model)
     param = lcfg.init_param(base.cfg.replace(sph_model=model))
@@ -660,3 +660,42 @@ def test_mixed_request_runs_fp64_for_other_models(model):
         da, db = a.download(), b.download()
         for k in ("x", "y", "u", "v", "rho", "pre"):
             assert np.array_equal(da[k], db[k]), k
+
+
+@pytest.mark.parametrize("model,ppw", [(6, 1), (6, 2)])
+@pytest.mark.parametrize("box", ["periodic", "walls"])
+def test_wave_particles_do_not_change_results(model, ppw, box):
+    """SPH_MODEL 6 with 1 or 2 particles per wavefront (lane groups of 64
+    or 32): the same results bit for bit as the default setting, over two
+    RK4 + PST steps (walls: the ghost fill in between)."""
+    from lssph import config as lcfg
+    if box == "periodic":
+        base, st = workloads.taylor_green_periodic(n=28, re=100.0, rk=4, jitter=0.2, seed=70 + model)
+    else:
+        base, st = workloads.taylor_green_box(n=24, re=100.0, rk=4, jitter=0.2, seed=70 + model)
+    param = lcfg.init_param(base.cfg.replace(sph_model=model))
+    print(f"[gpu] SPH_MODEL {model}, {ppw} particles per wavefront, {box}", flush=True)
+    out = []
+    for setting in (0, ppw):
+        with simulation_from_state(param, st, precision=0) as s:
+            s.set_wave_particles(setting)
+            if box == "walls":
+                s.ghost()
+            rhs = s.calculate_RHS()
+            s.step(2)
+            out.append((rhs, s.download()))
+    for k in ("u", "v", "rho"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    for k in ("x", "y", "u", "v", "rho", "pre"):
+        assert np.array_equal(out[0][1][k], out[1][1][k]), k
+
+
+def test_wave_particles_rejects_bad_values():
+    from lssph import config as lcfg
+    from lssph.engine import LssphError
+    base, st = workloads.taylor_green_periodic(n=16, re=100.0, rk=4, jitter=0.0, seed=0)
+    param = lcfg.init_param(base.cfg.replace(sph_model=6))
+    with simulation_from_state(param, st, precision=0) as s:
+        for bad in (3, 4, -1):  # 1 or 2 (0: the default)
+            with pytest.raises(LssphError):
+                s.set_wave_particles(bad)

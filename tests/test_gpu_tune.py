@@ -1,0 +1,73 @@
+"""The autotuner on the GPU, on small workloads: the setting it picks runs,
+and the settings that only change the layout on the GPU (threads per block,
+list length, particles per wavefront) give the same results bit for bit;
+the skin, which changes the summation order within stencil rows, moves the
+mixed path's FP32 sums by their rounding only (measured 9e-11 after 6
+steps; bound 1e-8)."""
+import numpy as np
+import pytest
+
+from helpers import rel_err
+from lssph import config as lcfg
+from lssph import tune, workloads
+from lssph.engine import Simulation, simulation_from_state, split_states
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(make, setting, steps):
+    sim = make(setting.skin)
+    try:
+        sim.set_tuning(block=setting.block, list_len=setting.list_len)
+        if setting.wave_particles:
+            sim.set_wave_particles(setting.wave_particles)
+        sim.step(steps)
+        return sim.download()
+    finally:
+        sim.close()
+
+
+def test_autotune_on_an_ensemble_and_its_pick_leaves_results_unchanged():
+    params, states, _ = workloads.cavity_ensemble(count=4, sizes=(24, 32, 48), seed=11, precision=1)
+    ni = [s.num_int for s in states]
+    ne = [s.num_ext for s in states]
+
+    def make(skin):
+        sim = Simulation(params, ni, ne, precision=1, skin=skin)
+        sim.upload(states)
+        sim.init_cell()
+        sim.ghost()
+        return sim
+
+    print("[gpu] autotune, 4-cavity ensemble", flush=True)
+    r = tune.autotune(make, steps=2, blocks=(64, 128, 256), skins=(0.06, 0.08, 0.12))
+    print(f"[gpu] tuned: {r.as_dict()['best']}", flush=True)
+    assert not r.best.error and np.isfinite(r.best.ms_per_step)
+    ok = [s for s in r.table if not s.error]
+    assert len(ok) >= 6
+    ref = _run(make, tune.Setting(128, 0, 0.08), 6)
+    for s in ok:
+        got = _run(make, s, 6)
+        for k in ("x", "y", "u", "v", "rho", "pre"):
+            if s.skin == 0.08:
+                assert np.array_equal(got[k], ref[k]), (s, k)
+            else:
+                assert rel_err(got[k], ref[k]) <= 1e-8, (s, k)
+
+
+def test_autotune_particles_per_wavefront_of_sph_model_6():
+    base, st = workloads.taylor_green_periodic(n=40, re=100.0, rk=4, jitter=0.2, seed=12)
+    param = lcfg.init_param(base.cfg.replace(sph_model=6))
+
+    def make(skin):
+        return simulation_from_state(param, st, precision=0, skin=skin)
+
+    print("[gpu] autotune, SPH_MODEL 6, particles per wavefront 1 and 2", flush=True)
+    r = tune.autotune(make, steps=1, blocks=(128,), skins=(0.08,), wave_particles=(1, 2),
+                      default=(128, 0, 0.08, 1))
+    print(f"[gpu] tuned: {r.as_dict()}", flush=True)
+    assert {s.wave_particles for s in r.table if not s.error} == {1, 2}
+    a = _run(make, tune.Setting(128, 0, 0.08, 1), 2)
+    b = _run(make, tune.Setting(128, 0, 0.08, 2), 2)
+    for k in ("x", "y", "u", "v", "rho", "pre"):
+        assert np.array_equal(a[k], b[k]), k

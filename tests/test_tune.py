@@ -17,7 +17,13 @@ class FakeSim:
     def __init__(self, skin):
         self.skin = skin
         self.block, self.list_len = 128, 0
+        self.ppw = 0
         self.clock = 0.0
+
+    def set_wave_particles(self, ppw):
+        if ppw not in (0, 1, 2):
+            raise LssphError(-1, "lssph_set_wave_particles: 0, 1 or 2")
+        self.ppw = ppw
 
     def set_tuning(self, block=0, list_len=0, reorder_interval=0):
         if block == 256 and list_len == 64:
@@ -30,7 +36,7 @@ class FakeSim:
         if self.skin == 0.02:
             raise LssphError(-4, "a particle moved farther than half the cell-list skin")
         cost = {64: 1.3, 128: 1.0, 256: 1.1}[self.block] * (1.0 + abs(self.skin - 0.08)) * (
-            1.05 if self.list_len == 64 else 1.0)
+            1.05 if self.list_len == 64 else 1.0) * (0.8 if self.ppw == 2 else 1.0)
         self.clock += cost * n
 
     def synchronize(self):
@@ -81,3 +87,15 @@ def test_autotune_skips_a_skin_that_fails_and_reports_it(fake_clock):
 def test_autotune_raises_when_nothing_runs(fake_clock):
     with pytest.raises(RuntimeError):
         tune.autotune(fake_clock, steps=2, blocks=(64,), skins=(0.02,))
+
+
+def test_autotune_covers_particles_per_wavefront(fake_clock):
+    """The SPH_MODEL 6 knob is one more dimension; an invalid value is
+    recorded as rejected; the default entry matches on all four settings."""
+    r = tune.autotune(fake_clock, steps=2, blocks=(128,), skins=(0.08,), wave_particles=(1, 2, 4),
+                      default=(128, 0, 0.08, 1))
+    assert r.best.wave_particles == 2
+    assert r.default.wave_particles == 1 and r.default.ms_per_step == pytest.approx(1.0)
+    bad = [s for s in r.table if s.error]
+    assert [s.wave_particles for s in bad] == [4]
+    assert r.as_dict()["best"]["wave_particles"] == 2
