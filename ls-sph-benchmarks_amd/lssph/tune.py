@@ -8,7 +8,9 @@ sets:
   cell-list skin (lssph_set_skin): cells of another width put the neighbors
       of a stencil row in another order, so the sums round differently (the
       FP64 path's trajectories stay within 1e-12 of each other, the mixed
-      path's within the FP32 rounding of its sums, ~1e-10 over a few steps).
+      path's within the FP32 rounding of its sums, i.e. within its own
+      error against an FP64 run: up to ~5e-7 of a field's maximum over a
+      few steps of a starting cavity flow).
 `autotune` times a few graph-replayed steps per candidate on the caller's own
 workload and returns the fastest.  No reference counterpart: the reference
 has no launch settings.
