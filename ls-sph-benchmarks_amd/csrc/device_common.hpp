@@ -42,7 +42,7 @@ struct InstDev {
   double sp_mass, w_ave, wl_thick, len_x, len_y;
   double inv_cw_x, inv_cw_y;  // internal cell grid: 1 / cell width (>= h_eff + skin)
   double inv_rcw_x, inv_rcw_y;  // periodic reference grid: cells / length
-  double skin_half_sq;        // (skin / 2)^2: largest squared move between cell builds
+  double skin_half_sq;        // (0.45 skin)^2: largest squared move between cell builds
   double w_norm;      // kernel normalisation, e.g. 7/(4 pi h^2) for Wendland C2
   double dw_fac;      // dW/dr = w_norm * shape' * dw_fac
   double rho_ref, vis_ref, c_sq, thermal_dif, alpha_ref, gravity, tem_ave;
@@ -72,6 +72,15 @@ struct Options {
 // FP32 pair terms, whose accuracy is set by the FP32 sums anyway.
 __device__ __forceinline__ float fast_sqrtf(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fast_rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// |(xs, ys)| of the mixed path's scaled offsets with the fused multiply-add
+// written out: left to FMA contraction, xs * xs + ys * ys may fuse either
+// product, and the choice can differ between the inlined copies of a pair
+// body (the walk has one per list layout), which would make a particle's
+// FP32 sums depend on the path its wavefront took.
+__device__ __forceinline__ float radius_s(float xs, float ys) {
+  return __builtin_amdgcn_sqrtf(__builtin_fmaf(xs, xs, ys * ys));
+}
 
 // ---------------------------------------------------------------------------
 // Kernel shapes (kernel_functions_mod.f90:26-146) as functions of q = r/h

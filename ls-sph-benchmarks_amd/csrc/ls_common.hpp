@@ -383,7 +383,7 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
   walk_neighbors<true>(A.mir, A.cstart, I, lc, s, me, L,
                  [&](const Rec &nb, double dx, double dy, int) {
                    const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
-                   const float q = fast_sqrtf(xs * xs + ys * ys);
+                   const float q = radius_s(xs, ys);
                    // the common factor w_norm of M and b cancels in the solve
                    const float vw = float(nb.vol) * kshape_w<float, KT, true>(kt, q);
                    float a[5];

@@ -261,7 +261,7 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
                      S.add(vj * w, a, df);
                    } else {
                      const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
-                     const float q = fast_sqrtf(xs * xs + ys * ys);
+                     const float q = radius_s(xs, ys);
                      const float vol = float(nb.vol);
                      float shp, g;
                      if constexpr (KT == 3) {
