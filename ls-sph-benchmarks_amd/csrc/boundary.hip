@@ -9,11 +9,14 @@
 //
 // Several lanes per marker shorten each lane's serial chain of dependent
 // loads and put more waves in flight when the markers are few; too many lose
-// to the group sums and the repeated solves.  The launcher takes 2 lanes
-// from 64 K markers up (100 K markers of a 64-cavity ensemble, WALL_MODEL 2:
-// 1, 2, 4, 8 lanes take 69, 49, 51, 58 us per launch) and 4 below (16 K
-// markers of 1024^2 Boussinesq convection, WALL_MODEL 3: 2, 4, 8 lanes take
-// 51, 31, 35 us).
+// to the group sums and the repeated solves.  The launcher takes 2 lanes for
+// the q = 3 fit of WALL_MODEL 2 (100 K markers of a 64-cavity ensemble: 1,
+// 2, 4, 8 lanes take 69, 49, 51, 58 us per launch) and 4 for the others
+// (16 K markers of 1024^2 Boussinesq convection, WALL_MODEL 3: 2, 4, 8
+// lanes take 51, 31, 35 us).  The lane count sets the order in which a
+// marker's partial sums are added; it follows from the wall model alone, a
+// handle-wide option, so the wall values of an instance do not depend on
+// the size of the ensemble it runs in.
 #include "pair_common.hpp"
 
 namespace lssph {
@@ -491,7 +494,7 @@ void launch_ghost(hipStream_t s, Device &d, int cur) {
   a.vm_tem = d.vm_tem;
   a.opt = d.opt;
   a.n_ext = d.n_ext;
-  if (d.n_ext >= 65536)
+  if (d.opt.wall_model == 2)
     ghost_launch<2>(s, d.opt.wall_model, d.n_ext, a);
   else
     ghost_launch<4>(s, d.opt.wall_model, d.n_ext, a);
