@@ -51,6 +51,13 @@ contains
         call execute_command_line("mkdir -p " // trim(adjustl(path)))
     end subroutine mkdir
 
+    subroutine copy_config(config_dir)
+        character(len=*), intent(in) :: config_dir
+        logical :: present
+        inquire(file="config.h", exist=present)
+        if (present) call execute_command_line("cp -f config.h " // trim(adjustl(config_dir)) // "/")
+    end subroutine copy_config
+
     ! Directories, parameters.csv, cell_idx.dat, the initial ghost fill and
     ! VM.dat (write_param_mod.f90:39-161).
     subroutine write_param(param, SP, VM, cell)
@@ -66,6 +73,9 @@ contains
         call mkdir(base_dir)
         call mkdir(config_dir)
         call mkdir(data_dir)
+        ! the run's config.h beside its parameters (write_param_mod.f90:50-53,
+        ! which copies it from the working directory)
+        call copy_config(config_dir)
 
         open(newunit=un, file=trim(config_dir) // "/parameters.csv", status="replace", form="formatted")
         write(un, "(a)") "Parameter, Value"
@@ -171,10 +181,30 @@ contains
             ", precision ", param%precision
     end subroutine write_init_info
 
+    ! Progress bar on the terminal and appended to results/<SAVE_NAME>/config/
+    ! run.log, in the reference's format (write_progress_mod.f90:22-69): the
+    ! share of the run's steps done (counted from the restart step for a
+    ! restart) and the wall time since the loop began.
     subroutine write_progress(step, param, seconds)
         integer         , intent(in) :: step
         type(Param_type), intent(in) :: param
         double precision, intent(in) :: seconds
-        write(*, '(a, i0, a, i0, a, f10.3, a)') "   step ", step, " / ", param%end_step, "  (", seconds, " s)"
+        character(len=20)   :: bar
+        character(len=1024) :: log_file
+        character(len=96)   :: line
+        integer :: done, filled, un
+        double precision :: frac
+        done = step
+        if (trim(adjustl(param%read_name)) /= "new") done = step - param%start_step
+        frac = dble(done) / dble(max(param%total_step, 1))
+        filled = max(0, min(20, int(frac * 20.0d0)))
+        bar = repeat("=", filled) // repeat(" ", 20 - filled)
+        write(line, "(a, a, a, f6.2, a, es11.4, a)") "   Progress: [", bar, "] ", frac * 100.0d0, &
+            "%  |  Elapsed Time: ", seconds, " [s]"
+        write(*, "(a)") trim(line)
+        log_file = "results/" // trim(adjustl(param%save_name)) // "/config/run.log"
+        open(newunit=un, file=trim(log_file), status="unknown", position="append", form="formatted")
+        write(un, "(a)") trim(line)
+        close(un)
     end subroutine write_progress
 end module write_progress_mod

@@ -129,6 +129,7 @@ contains
     ! A restart writes into results/SAVE_NAME/data as well.
     subroutine write_param_dirs()
         call execute_command_line("mkdir -p results/" // trim(adjustl(param%save_name)) // "/data")
+        call execute_command_line("mkdir -p results/" // trim(adjustl(param%save_name)) // "/config")
     end subroutine write_param_dirs
 
 end program main

@@ -17,7 +17,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from fortran_driver import read_parameters, read_state, run, write_state
+from fortran_driver import REPO, read_parameters, read_state, run, write_state
 from helpers import Golden, rel_err
 
 CASES_DIR = Path(__file__).resolve().parent / "fortran_cases"
@@ -99,11 +99,20 @@ def _compare(got: dict, ref: Golden, tag: str, label: str, tol=None):
 @pytest.mark.gpu
 def test_driver_cavity_new_run(tmp_path):
     """Lid-driven cavity, RK2, eager library calls: initial output, per-step
-    output, parameters.csv, cell_idx.dat and VM.dat against the reference."""
+    output, parameters.csv, cell_idx.dat and VM.dat against the reference,
+    and its side outputs: the working directory's config.h copied beside
+    the parameters and the progress log run.log (write_param_mod.f90:50-53,
+    write_progress_mod.f90:53-69)."""
     g = Golden("cavity_rk2")
+    header = (REPO / "tests" / "fortran_cases" / "cavity_rk2.h").read_text()
+    (tmp_path / "config.h").write_text(header)
     r = run("cavity_rk2", tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     base = tmp_path / "results" / save_name("cavity_rk2")
+    assert (base / "config" / "config.h").read_text() == header
+    log = (base / "config" / "run.log").read_text().splitlines()
+    assert log and all(line.startswith("   Progress: [") for line in log), log[:3]
+    assert "] 100.00%  |  Elapsed Time: " in log[-1], log[-1]
     data = base / "data"
     _compare(read_state(data / "0.dat", g.n_total, g.has_tem), g, "state0", "0.dat")
     for k, tag in ((1, "step_1"), (2, "step_2"), (g.steps, "step_last")):
