@@ -70,10 +70,10 @@ class HipEvent:
         return float(ms.value)
 
 
-def run_gpu(param, st, precision: int, steps: int, warmup: int, setup=None) -> dict:
+def run_gpu(param, st, precision: int, steps: int, warmup: int, setup=None, cell_split=None) -> dict:
     print(f"[bench] precision={'mixed' if precision else 'fp64'}: upload {st.num_int} particles", flush=True)
     t_up = time.perf_counter()
-    sim = simulation_from_state(param, st, precision=precision)
+    sim = simulation_from_state(param, st, precision=precision, cell_split=cell_split)
     if setup:
         setup(sim)
     sim.synchronize()
@@ -300,8 +300,8 @@ def ensemble_bench(args) -> None:
     ni = [st.num_int for st in states]
     ne = [st.num_ext for st in states]
 
-    def make(skin: float = 0.08):
-        sim = Simulation(params, ni, ne, precision=1, skin=skin)
+    def make(skin: float = 0.08, cell_split: int = 0):
+        sim = Simulation(params, ni, ne, precision=1, skin=skin, cell_split=cell_split or None)
         sim.set_isolation(True)
         sim.upload(states)
         sim.init_cell()
@@ -310,11 +310,11 @@ def ensemble_bench(args) -> None:
 
     print(f"[bench] ensemble: {len(states)} cavities, {sum(ni)} fluid + {sum(ne)} wall particles; autotuning",
           flush=True)
-    tuned = tune.autotune(make, steps=5)
+    tuned = tune.autotune(make, steps=5, splits=(0, 1, 3))
     b = tuned.best
 
     def make_best():
-        sim = make(b.skin)
+        sim = make(b.skin, b.cell_split)
         sim.set_tuning(block=b.block, list_len=b.list_len)
         return sim
 
@@ -419,6 +419,8 @@ def main():
                     help="1-2 classical SPH, 4-6 LS q=9/14/20: time that operator (FP64) instead of the headline")
     ap.add_argument("--wave-particles", type=int, default=0,
                     help="particles per wavefront of the SPH_MODEL 5-6 kernels (0: the library's default)")
+    ap.add_argument("--cell-split", type=int, default=0,
+                    help="cell columns per stencil reach in x (0: the library's default)")
     ap.add_argument("--workload", choices=("taylor-green", "ensemble", "boussinesq"), default="taylor-green",
                     help="ensemble: 64 lid-driven cavities (autotuned, graph-replayed); boussinesq: 1024^2 "
                          "convection, mixed vs FP64; each prints a line of its own, not the headline")
@@ -444,8 +446,9 @@ def main():
     n = args.n
     param, st = workloads.taylor_green_periodic(n=n, re=1000.0, rk=4, jitter=0.2, seed=2029)
     n_part = st.num_int
-    mixed = run_gpu(param, st, 1, args.steps, args.warmup)
-    fp64 = None if args.no_fp64 else run_gpu(param, st, 0, args.steps, args.warmup)
+    split = args.cell_split or None
+    mixed = run_gpu(param, st, 1, args.steps, args.warmup, cell_split=split)
+    fp64 = None if args.no_fp64 else run_gpu(param, st, 0, args.steps, args.warmup, cell_split=split)
 
     value = n_part * args.steps / (mixed["ms"] / 1e3)
     kt = mixed["kt"]

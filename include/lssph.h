@@ -291,6 +291,17 @@ int32_t lssph_set_tuning(lssph_handle_t h, int32_t block, int32_t list_len,
    does.  Call before lssph_upload. */
 int32_t lssph_set_skin(lssph_handle_t h, double skin);
 
+/* Cell columns per stencil reach in x, 1 to 4 (default 2; no reference
+   counterpart: the reference's init_cell_mod.f90:36-41 grid is square, as
+   with 1).  The internal cells stay h_eff * (1 + skin) tall and become
+   h_eff * (1 + skin) / columns wide; a particle's stencil is the
+   2 columns + 1 cells of the rows above, at and below its cell, which holds
+   about (2 columns + 1) / (3 columns) of the candidates of a square 3x3
+   stencil.  The neighbor sets do not change; the order of the candidates
+   within a stencil row does, so the sums round differently (as with another
+   skin).  Call before lssph_upload. */
+int32_t lssph_set_cell_split(lssph_handle_t h, int32_t columns);
+
 /* Candidate scan of the pair kernels (no reference counterpart).  1 (the
    default): a full wavefront whose particles lie in one cell row loads its
    candidates' positions once into on-chip memory and scans them there;

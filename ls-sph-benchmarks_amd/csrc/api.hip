@@ -26,6 +26,7 @@ struct lssph_handle {
   bool spd_fresh = false;  // Device::spd holds the speeds of the current slots
   int max_passes = 1;   // passes a cell list serves (rk + 1 with a skin, else 1)
   double skin = 0.08;   // cell-grid skin as a fraction of h_eff
+  int cell_split = 2;   // internal cell columns per stencil reach in x (InstDev::kx)
   bool uploaded = false, cells_ready = false;
   std::string err;
   // one captured step per starting state (records buffer, build buffers,
@@ -148,7 +149,7 @@ double cell_width(const lssph_params_t &p, double skin) {
   return std::max(w, p.h_eff);
 }
 
-InstDev make_inst(const lssph_params_t &p, double skin) {
+InstDev make_inst(const lssph_params_t &p, double skin, int kx) {
   InstDev I{};
   I.h = p.h;
   I.inv_h = 1.0 / p.h;
@@ -197,8 +198,10 @@ InstDev make_inst(const lssph_params_t &p, double skin) {
   I.periodic = p.periodic;
   const double w = cell_width(p, skin);
   double sk = w - p.h_eff;  // the skin this instance gets
+  // cells are w tall and w / kx wide (kx columns per stencil reach)
+  I.kx = kx;
   if (p.periodic) {
-    I.nx = (int)std::floor(p.len_x / w);
+    I.nx = (int)std::min(kx * std::floor(p.len_x / w), 2e9);  // > 2^30 cells fails setup_grid's check
     I.ny = (int)std::floor(p.len_y / w);
     I.inv_cw_x = (double)I.nx / p.len_x;
     I.inv_cw_y = (double)I.ny / p.len_y;
@@ -206,12 +209,12 @@ InstDev make_inst(const lssph_params_t &p, double skin) {
     I.rny = (int)std::floor(p.len_y / p.h_eff);
     I.inv_rcw_x = (double)I.rnx / p.len_x;
     I.inv_rcw_y = (double)I.rny / p.len_y;
-    // whole cells per period: the cells are at least w wide
-    sk = std::min(p.len_x / I.nx, p.len_y / I.ny) - p.h_eff;
+    // whole cells per period: a stencil reaches at least w
+    sk = std::min(p.len_x / (I.nx / kx), p.len_y / I.ny) - p.h_eff;
   } else {
-    I.nx = (int)std::floor((p.len_x + 2.0 * p.wl_thick) / w) + 1;
+    I.nx = (int)std::min(std::floor((p.len_x + 2.0 * p.wl_thick) * kx / w) + 1.0, 2e9);
     I.ny = (int)std::floor((p.len_y + 2.0 * p.wl_thick) / w) + 1;
-    I.inv_cw_x = 1.0 / w;
+    I.inv_cw_x = kx / w;
     I.inv_cw_y = 1.0 / w;
     // init_cell_mod.f90:36-41
     I.rnx = (int)std::ceil((p.len_x + 2.0 * p.wl_thick) / p.h_eff);
@@ -411,7 +414,7 @@ int64_t setup_grid(lssph_handle *h) {
   int64_t nc = 0;
   bool all_skin = true;
   for (size_t k = 0; k < h->params.size(); ++k) {
-    InstDev I = make_inst(h->params[k], h->skin);
+    InstDev I = make_inst(h->params[k], h->skin, h->cell_split);
     I.cell_off = (int32_t)nc;
     nc += (int64_t)I.nx * I.ny;
     all_skin = all_skin && I.skin_half_sq > 0.0;
@@ -551,6 +554,8 @@ int32_t lssph_create(const lssph_params_t *params, int32_t n_inst, const int32_t
   rc = rc ? rc : dalloc(h, &d.inst, n_inst);
   rc = rc ? rc : dalloc(h, &d.mir[0], n);
   rc = rc ? rc : dalloc(h, &d.mir[1], n);
+  if (mixed_path(d))
+    for (int b = 0; b < 2 && !rc; ++b) rc = dalloc(h, &d.nbf[b], n);
   rc = rc ? rc : dalloc(h, &d.xbuild, 2 * n);
   rc = rc ? rc : dalloc(h, &d.info, n);
   rc = rc ? rc : dalloc(h, &d.key, n);
@@ -1049,6 +1054,23 @@ int32_t lssph_set_skin(lssph_handle_t h, double skin) {
     h->skin = old;
     setup_grid(h);
     return fail(h, LSSPH_E_ARG, "lssph_set_skin: problem too large for 32-bit indices at this skin");
+  }
+  drop_graphs(h);
+  if (int rc = alloc_grid(h)) return rc;
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+int32_t lssph_set_cell_split(lssph_handle_t h, int32_t columns) {
+  if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
+  if (columns < 1 || columns > 4) return fail(h, LSSPH_E_ARG, "lssph_set_cell_split: columns must be 1 to 4");
+  if (h->uploaded) return fail(h, LSSPH_E_STATE, "lssph_set_cell_split: call before lssph_upload");
+  const int old = h->cell_split;
+  h->cell_split = columns;
+  if (setup_grid(h) > (1ll << 30)) {
+    h->cell_split = old;
+    setup_grid(h);
+    return fail(h, LSSPH_E_ARG, "lssph_set_cell_split: problem too large for 32-bit indices with these cells");
   }
   drop_graphs(h);
   if (int rc = alloc_grid(h)) return rc;

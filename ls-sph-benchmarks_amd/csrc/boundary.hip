@@ -23,6 +23,7 @@ namespace lssph {
 
 struct GhostArgs {
   Rec *__restrict__ mir;   // current records: fluid read, wall slots written
+  float4 *__restrict__ nbf;  // their FP32 neighbor fields (mixed path; else null)
   const int32_t *__restrict__ sinfo;
   const int32_t *__restrict__ slot_of;
   const int32_t *__restrict__ cstart;
@@ -311,7 +312,7 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs A) {
         break;
     }
   }
-  A.mir[sw] = w;
+  store_rec(A.mir, A.nbf, sw, w);
 }
 
 // Virtual markers from the wall positions (init_virtual_markers_mod.f90:
@@ -476,6 +477,7 @@ void launch_ghost(hipStream_t s, Device &d, int cur) {
   if (d.n_ext == 0) return;
   GhostArgs a;
   a.mir = d.mir[cur];
+  a.nbf = d.nbf[cur];
   a.sinfo = d.sinfo;
   a.slot_of = d.slot_of;
   a.cstart = d.cell_start;

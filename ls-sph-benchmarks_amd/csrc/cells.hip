@@ -9,8 +9,8 @@
 //     per-cell insertion sort;
 //   incremental build (after every RK stage and shift): the kernel that
 //     moved the particles has written each slot's new cell key; each new
-//     cell gathers its members from the previous build's 3x3 stencil of
-//     cells (particles move far less than a cell per stage), counts them,
+//     cell gathers its members from the previous build's stencil of cells
+//     (particles move far less than a cell per stage), counts them,
 //     and after the scan writes them in order: the members that stayed are
 //     already sorted, the few arrivals are merged in.  No atomics, no sort.
 #include <climits>
@@ -395,21 +395,22 @@ k_cell_fill(int ncells, const int32_t *__restrict__ cell_inst, const InstDev *__
 }
 
 // Records in the new slot order: dst[t] = src[previous slot of the particle
-// at t], with the info, cell key and upload index of each slot and the
-// position the cell list is built at.  slot_of[p] turns from the previous
-// slot into the new one (each thread reads and writes only its own p).
-// Also empties the stray list for the next moves.
+// at t] (and their FP32 neighbor fields when nbf_dst is non-null), with the
+// info, cell key and upload index of each slot and the position the cell
+// list is built at.  slot_of[p] turns from the previous slot into the new one
+// (each thread reads and writes only its own p).  Also empties the stray
+// list for the next moves.
 __global__ void k_mirror(int n, const int32_t *__restrict__ items, int32_t *__restrict__ slot_of,
                          const int32_t *__restrict__ info, const int32_t *__restrict__ key,
                          const int32_t *__restrict__ orig_of, const Rec *__restrict__ src, Rec *__restrict__ dst,
-                         int32_t *__restrict__ sinfo, int32_t *__restrict__ kslot, int32_t *__restrict__ sorig,
-                         double *__restrict__ xbuild, int32_t *__restrict__ nstray) {
+                         float4 *__restrict__ nbf_dst, int32_t *__restrict__ sinfo, int32_t *__restrict__ kslot,
+                         int32_t *__restrict__ sorig, double *__restrict__ xbuild, int32_t *__restrict__ nstray) {
   const int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t == 0) *nstray = 0;
   if (t >= n) return;
   const int p = items[t];
   const Rec r = src[slot_of[p]];
-  dst[t] = r;
+  store_rec(dst, nbf_dst, t, r);
   sinfo[t] = info[p];
   kslot[t] = key[p];
   sorig[t] = orig_of[p];
@@ -467,8 +468,8 @@ void launch_exclusive_scan(hipStream_t s, const int32_t *in, int32_t *out, int32
 
 static void launch_mirror(hipStream_t s, Device &d, int src, int dst) {
   hipLaunchKernelGGL(k_mirror, dim3(xcd_grid(d.n_total, 256)), dim3(256), 0, s, d.n_total, d.items, d.slot_of,
-                     d.info, d.key, d.orig_of, d.mir[src], d.mir[dst], d.sinfo, d.kslot, d.sorig, d.xbuild,
-                     d.nstray);
+                     d.info, d.key, d.orig_of, d.mir[src], d.mir[dst], d.nbf[dst], d.sinfo, d.kslot, d.sorig,
+                     d.xbuild, d.nstray);
 }
 
 void launch_bin(hipStream_t s, Device &d, int src, int dst, int fixed_key_mode) {

@@ -26,6 +26,22 @@ struct alignas(64) Rec {
   double rho, vol, pre, tem;
 };
 
+// The neighbor fields of the mixed-precision path: (u, v, pre, vol) of a
+// record rounded to FP32, 16 B per slot in an array beside the records (the
+// same slot order and double buffering).  A mixed pair walk gathers these
+// instead of two 16-B chunks of the 64-B record: one load per neighbor, and
+// eight neighbors per 128-B cache line instead of two.
+__device__ __forceinline__ float4 nb_fields(const Rec &r) {
+  return make_float4(float(r.u), float(r.v), float(r.pre), float(r.vol));
+}
+
+// Writes record r to slot s, and its neighbor fields when the handle keeps
+// them (nbf non-null: the mixed path).
+__device__ __forceinline__ void store_rec(Rec *__restrict__ out, float4 *__restrict__ nbf, int s, const Rec &r) {
+  out[s] = r;
+  if (nbf) nbf[s] = nb_fields(r);
+}
+
 // Packed per-particle info: ptype in bits 0..7 (0 = fluid), instance id in
 // bits 8..31.
 __host__ __device__ inline int info_ptype(int32_t i) { return i & 0xff; }
@@ -55,7 +71,10 @@ struct InstDev {
   int32_t nx, ny, cell_off, periodic;  // internal cell grid
   int32_t rnx, rny;                     // the reference's cell grid (cell_idx output)
   int32_t retired;  // isolation mode: set (by a vector store) when the instance is retired
-  int32_t pad1;
+  // Cell columns per stencil reach in x: the internal cells are (h_eff +
+  // skin) / kx wide in x and h_eff + skin tall, and the stencil of a cell is
+  // the 2 kx + 1 columns of the rows above, at and below it.
+  int32_t kx;
 };
 
 // Model options shared by every instance of a handle.

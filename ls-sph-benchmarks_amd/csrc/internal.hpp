@@ -31,7 +31,7 @@ constexpr int kSlotBits = 27;
 constexpr int kSlotMask = (1 << kSlotBits) - 1;
 constexpr long long kMaxParticles = kSlotMask;
 
-// Incremental cell rebuild: particles whose new cell is outside the 3x3
+// Incremental cell rebuild: particles whose new cell is outside the
 // stencil of their old cell are listed (old slot, new key) for the rebuild;
 // more than kStrayCap of them in one stage is reported as an error.
 constexpr int kStrayCap = 4096;
@@ -47,6 +47,7 @@ struct Device {
   Options opt{};
   InstDev *inst = nullptr;          // [n_inst]
   Rec *mir[2] = {nullptr, nullptr};  // records in slot order, double-buffered across passes
+  float4 *nbf[2] = {nullptr, nullptr};  // mixed path only: nb_fields of mir[b], same slots
   int32_t *info = nullptr;          // [n_total] ptype | inst << 8
   int32_t *key = nullptr;           // [n_total] global cell id
   int32_t *orig_of = nullptr;       // [n_total] physical -> upload index
@@ -101,6 +102,10 @@ struct Device {
   int wave_ppw = 0;      // particles per wavefront of the SPH_MODEL 6 kernels (0: default)
   int red_blocks = 256;  // blocks per instance of the max reductions
 };
+
+// The mixed-precision path runs (and the FP32 neighbor fields nbf are kept)
+// for SPH_MODEL 3 handles created with precision LSSPH_MIXED.
+inline bool mixed_path(const Device &d) { return d.opt.precision == LSSPH_MIXED && d.opt.sph_model == 3; }
 
 // Particles per wavefront of the lane-group kernels of SPH_MODEL 6 (1 or
 // 2); the default is the faster setting measured at 1024^2.

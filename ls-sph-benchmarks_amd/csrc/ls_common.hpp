@@ -24,6 +24,22 @@ struct Layout {
   static constexpr int RHO = NF - 1;  // valid when DELTA
 };
 
+// The mixed path's differences: u, v and pre from the FP32 neighbor fields
+// (g of the neighbor, own of the particle), tem and the delta-SPH rho from
+// the FP64 records (the neighbor fields hold four values).
+template <int CLS, bool DELTA>
+__device__ __forceinline__ void field_diffs32(const float4 &g, const float4 &own, const Rec &nb, const Rec &me,
+                                              float (&df)[Layout<CLS, DELTA>::NF]) {
+  using L = Layout<CLS, DELTA>;
+  df[L::U] = g.x - own.x;
+  if constexpr (CLS >= 1) {
+    df[L::V] = g.y - own.y;
+    df[L::P] = g.z - own.z;
+    if constexpr (CLS == 2) df[L::T] = float(nb.tem - me.tem);
+    if constexpr (DELTA) df[L::RHO] = float(nb.rho - me.rho);
+  }
+}
+
 template <int CLS, bool DELTA, typename R>
 __device__ __forceinline__ void field_diffs(const Rec &nb, const Rec &me, R (&df)[Layout<CLS, DELTA>::NF]) {
   using L = Layout<CLS, DELTA>;
@@ -308,6 +324,8 @@ __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, do
 struct LsaArgs {
   const Rec *__restrict__ mir;        // records in slot order (read)
   Rec *__restrict__ mout;             // records in slot order (write)
+  const float4 *__restrict__ nbf;     // mixed path: FP32 neighbor fields of mir (else null)
+  float4 *__restrict__ nbf_out;       // mixed path: those of mout (else null)
   const int32_t *__restrict__ sinfo;  // info in slot order
   const int32_t *__restrict__ items;  // slot -> physical id
   const int32_t *__restrict__ orig_of;  // physical id -> upload index
@@ -372,6 +390,8 @@ __device__ __forceinline__ void derivs64(const LsaArgs &A, const InstDev &I, int
 }
 
 // Mixed-precision assembly + FP64 LDL^T with refinement; false -> fallback.
+// Offsets come from FP64 positions (x_j - x_i scaled by 1/h, then rounded);
+// the neighbor's u, v, pre and volume are gathered as FP32 (nbf).
 template <int CLS, bool DELTA, int KT>
 __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int lc, int s, const Rec &me,
                                          const NbList &L,
@@ -380,16 +400,18 @@ __device__ __forceinline__ bool derivs32(const LsaArgs &A, const InstDev &I, int
   const int kt = A.kt;
   LsSums<float, NF> S;
   S.zero();
+  const float4 own = nb_fields(me);
   walk_neighbors<true>(A.mir, A.cstart, I, lc, s, me, L,
-                 [&](const Rec &nb, double dx, double dy, int) {
+                 [&](const Rec &nb, double dx, double dy, int t) {
                    const float xs = float(dx * I.inv_h), ys = float(dy * I.inv_h);
                    const float q = radius_s(xs, ys);
+                   const float4 g = A.nbf[t];
                    // the common factor w_norm of M and b cancels in the solve
-                   const float vw = float(nb.vol) * kshape_w<float, KT, true>(kt, q);
+                   const float vw = g.w * kshape_w<float, KT, true>(kt, q);
                    float a[5];
                    basis5s(xs, ys, a);
                    float df[NF];
-                   field_diffs<CLS, DELTA>(nb, me, df);
+                   field_diffs32<CLS, DELTA>(g, own, nb, me, df);
                    S.add_reduced(vw, a, df);
                  });
   S.complete();
@@ -535,7 +557,7 @@ __device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p
     out.vol = I.sp_mass / out.rho;
     if (A.spd) A.spd[s] = sqrt(out.u * out.u + out.v * out.v);  // PST_mod.f90:51
   }
-  A.mout[s] = out;
+  store_rec(A.mout, A.nbf_out, s, out);
   if (!DIFF) record_move(A.mv, I, s, out.x, out.y);
 }
 
@@ -543,6 +565,8 @@ __device__ __forceinline__ void finish(const LsaArgs &A, const InstDev &I, int p
 struct PstArgs {
   const Rec *__restrict__ mir;
   Rec *__restrict__ mout;
+  const float4 *__restrict__ nbf;     // mixed path: FP32 neighbor fields of mir (else null)
+  float4 *__restrict__ nbf_out;       // mixed path: those of mout (else null)
   const int32_t *__restrict__ sinfo;
   const int32_t *__restrict__ items;
   const int32_t *__restrict__ orig_of;

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256)
   const int32_t inf = A.sinfo[s];
   // walls (and the particles of retired instances) move on with the records
   auto keep = [&] {
-    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];
+    if (A.mode != kRhsOnly) store_rec(A.mout, A.nbf_out, s, A.mir[s]);
     if (A.spd) A.spd[s] = 0.0;
   };
   if (info_ptype(inf) != 0) {
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_csph(LsaArgs A) {
   if (s >= A.n) return;
   const int32_t inf = A.sinfo[s];
   auto keep = [&] {
-    if (A.mode != kRhsOnly) A.mout[s] = A.mir[s];
+    if (A.mode != kRhsOnly) store_rec(A.mout, A.nbf_out, s, A.mir[s]);
     if (A.spd) A.spd[s] = 0.0;
   };
   if (info_ptype(inf) != 0) {
@@ -237,6 +237,9 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
   // others would spill under the 128-register cap)
   walk_neighbors<MIXED && !T && KT == 3>(A.mir, A.cstart, I, lc, s, me, nl,
                  [&](const Rec &nb, double dx, double dy, int) {
+                   // the shifting gathers (u, v) and (rho, vol) from the
+                   // record, one cache line per neighbor: its fit needs rho,
+                   // which the FP32 neighbor fields do not hold
                    R df[NF];
                    df[0] = R(nb.u - me.u);
                    df[1] = R(nb.v - me.v);
@@ -349,7 +352,7 @@ __device__ __forceinline__ void pst_particle(const PstArgs &A, const InstDev &I,
   if constexpr (T) out.tem = me2.tem + acc[3];
   out.pre = I.c_sq * (out.rho - I.rho_ref);
   out.vol = I.sp_mass / out.rho;
-  A.mout[s] = out;
+  store_rec(A.mout, A.nbf_out, s, out);
   record_move(A.mv, I, s, out.x, out.y);
   }
 }
@@ -366,7 +369,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED 
   const int s = xcd_block() * B + tid;
   if (s >= A.n) return;
   const int32_t inf = A.sinfo[s];
-  auto keep = [&] { A.mout[s] = A.mir[s]; };  // walls move on with the records
+  auto keep = [&] { store_rec(A.mout, A.nbf_out, s, A.mir[s]); };  // walls move on with the records
   if (info_ptype(inf) != 0) {
     keep();
     return;
@@ -548,6 +551,8 @@ void launch_lsa(hipStream_t s, Device &d, int cur, StageMode mode, int guard, in
   a.spd = write_spd ? d.spd : nullptr;
   a.mir = d.mir[cur];
   a.mout = d.mir[cur ^ 1];
+  a.nbf = d.nbf[cur];
+  a.nbf_out = d.nbf[cur ^ 1];
   a.sinfo = d.sinfo;
   a.items = d.items;
   a.orig_of = d.orig_of;
@@ -630,6 +635,7 @@ void launch_pair_lists(hipStream_t s, Device &d, int cur, int32_t *counts, int32
                        int cap) {
   LsaArgs a{};
   a.mir = d.mir[cur];
+  a.nbf = d.nbf[cur];
   a.sinfo = d.sinfo;
   a.items = d.items;
   a.orig_of = d.orig_of;
@@ -678,6 +684,8 @@ void launch_pst(hipStream_t s, Device &d, int cur, int guard, int keys) {
   PstArgs a;
   a.mir = d.mir[cur];
   a.mout = d.mir[cur ^ 1];
+  a.nbf = d.nbf[cur];
+  a.nbf_out = d.nbf[cur ^ 1];
   a.sinfo = d.sinfo;
   a.items = d.items;
   a.orig_of = d.orig_of;

@@ -2,10 +2,11 @@
 //
 // Neighbor data comes from the records in slot order (cells consecutive,
 // each cell's particles in ascending upload index like the reference's
-// lists, update_cell_mod.f90:57-92), so the 3x3 stencil of a particle is at
-// most three contiguous slot ranges (four to six when a periodic row wraps)
-// and lanes of a wavefront read the same or adjacent cache lines.  Cells are
-// h_eff + skin wide and rebuilt once per time step; every pass still tests
+// lists, update_cell_mod.f90:57-92), so the stencil of a particle (three
+// cell rows of 2 kx + 1 columns) is at most three contiguous slot ranges
+// (four to six when a periodic row wraps) and lanes of a wavefront read the
+// same or adjacent cache lines.  Cells are h_eff + skin tall and (h_eff +
+// skin) / kx wide and rebuilt once per time step; every pass still tests
 // r_ij^2 <= h_eff^2 at the current positions, so the neighbor sets are the
 // reference's.
 #pragma once
@@ -88,16 +89,16 @@ __device__ __forceinline__ double2 load_xy(const Rec *__restrict__ r, int j) {
   return *reinterpret_cast<const double2 *>(&r[j].x);
 }
 
-// Visits the slot ranges of the 3x3 stencil around local cell `lc` in the
-// reference's order (rows dy = -1..1, within a row dx = -1..1), passing the
-// periodic image code of each range (x code in bits 0-1, y code in 2-3) and
-// its range id 2 * (dy + 1) + segment.  A row is one range, or two when a
-// periodic row wraps; the loops are kept rolled so the candidate loop body
-// exists once in the code.
+// Visits the slot ranges of the stencil around local cell `lc` (rows dy =
+// -1..1, within a row the columns dx = -kx..kx, the reference's order when
+// kx = 1), passing the periodic image code of each range (x code in bits
+// 0-1, y code in 2-3) and its range id 2 * (dy + 1) + segment.  A row is one
+// range, or two when a periodic row wraps; the loops are kept rolled so the
+// candidate loop body exists once in the code.
 template <typename F>
 __device__ __forceinline__ void visit_cells(const InstDev &I, int lc, const int32_t *__restrict__ cstart,
                                             F &&f) {
-  const int nx = I.nx, ny = I.ny;
+  const int nx = I.nx, ny = I.ny, kx = I.kx;
   const int cx = lc % nx, cy = lc / nx;
 #pragma unroll 1
   for (int dy = -1; dy <= 1; ++dy) {
@@ -116,24 +117,26 @@ __device__ __forceinline__ void visit_cells(const InstDev &I, int lc, const int3
     const int row = I.cell_off + yy * nx;
     int a0, a1, b0 = 0, b1 = 0, ca = 0, cb = 0;
     if (!I.periodic) {
-      const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < nx - 1 ? cx + 1 : nx - 1;
+      const int x0 = max(cx - kx, 0), x1 = min(cx + kx, nx - 1);
       a0 = cstart[row + x0];
       a1 = cstart[row + x1 + 1];
-    } else if (cx == 0) {
-      a0 = cstart[row + nx - 1];
+    } else if (cx < kx) {
+      // columns nx + cx - kx .. nx - 1 seen across the left edge, then 0 .. cx + kx
+      a0 = cstart[row + nx + cx - kx];
       a1 = cstart[row + nx];
       ca = 1;
       b0 = cstart[row];
-      b1 = cstart[row + 2];
-    } else if (cx == nx - 1) {
-      a0 = cstart[row + cx - 1];
-      a1 = cstart[row + cx + 1];
+      b1 = cstart[row + cx + kx + 1];
+    } else if (cx > nx - 1 - kx) {
+      // columns cx - kx .. nx - 1, then 0 .. cx + kx - nx across the right edge
+      a0 = cstart[row + cx - kx];
+      a1 = cstart[row + nx];
       b0 = cstart[row];
-      b1 = cstart[row + 1];
+      b1 = cstart[row + cx + kx + 1 - nx];
       cb = 2;
     } else {
-      a0 = cstart[row + cx - 1];
-      a1 = cstart[row + cx + 2];
+      a0 = cstart[row + cx - kx];
+      a1 = cstart[row + cx + kx + 1];
     }
 #pragma unroll 1
     for (int seg = 0; seg < 2; ++seg) {
@@ -143,12 +146,12 @@ __device__ __forceinline__ void visit_cells(const InstDev &I, int lc, const int3
   }
 }
 
-// True when the lane's 3x3 stencil wraps across a periodic boundary, i.e.
-// some candidate range carries an image shift.
+// True when the lane's stencil wraps across a periodic boundary, i.e. some
+// candidate range carries an image shift.
 __device__ __forceinline__ bool stencil_wraps(const InstDev &I, int lc) {
   if (!I.periodic) return false;
   const int cx = lc % I.nx, cy = lc / I.nx;
-  return cx == 0 || cx == I.nx - 1 || cy == 0 || cy == I.ny - 1;
+  return cx < I.kx || cx > I.nx - 1 - I.kx || cy == 0 || cy == I.ny - 1;
 }
 
 // True when any active lane of the wavefront has a wrapping stencil.  Waves
@@ -247,6 +250,7 @@ __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const i
   if (__builtin_amdgcn_read_exec() != ~0ull) return false;
   const int nx = __builtin_amdgcn_readfirstlane(I.nx);
   const int ny = __builtin_amdgcn_readfirstlane(I.ny);
+  const int kx = __builtin_amdgcn_readfirstlane(I.kx);
   const int off = __builtin_amdgcn_readfirstlane(I.cell_off);
   const int cx = lc % I.nx, cy = lc / I.nx;
   if (__ballot(I.cell_off != off || stencil_wraps(I, lc)) != 0ull) return false;
@@ -262,8 +266,8 @@ __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const i
   // a wave on two rows: four candidate rows, three of them per lane
   const int cy0 = __builtin_amdgcn_readfirstlane(ymn), cy1 = __builtin_amdgcn_readfirstlane(ymx);
   if (cy1 - cy0 > 1) return false;
-  const int x0 = max(__builtin_amdgcn_readfirstlane(mn) - 1, 0);
-  const int x1 = min(__builtin_amdgcn_readfirstlane(mx) + 1, nx - 1);
+  const int x0 = max(__builtin_amdgcn_readfirstlane(mn) - kx, 0);
+  const int x1 = min(__builtin_amdgcn_readfirstlane(mx) + kx, nx - 1);
   const int y0 = max(cy0 - 1, 0), y1 = min(cy1 + 1, ny - 1);
   const int cap = 8 * (L.list_len + 1);  // 16-byte positions in the upper half of the wave's rows
   const int base = cstart[off + y0 * nx + x0];
@@ -279,7 +283,7 @@ __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const i
   char *l16 = L.wbase + 2 * lane;
   double2 *stage = reinterpret_cast<double2 *>(L.wbase + 128 * (L.list_len + 1));
   const unsigned list_len = (unsigned)L.list_len;
-  const int lx0 = max(cx - 1, 0), lx1 = min(cx + 1, nx - 1);
+  const int lx0 = max(cx - kx, 0), lx1 = min(cx + kx, nx - 1);
   int cnt = 0;
   auto scan_window = [&](int a0, int a1, int shift, auto &&entry) {
     auto accept = [&](int t, double2 p) {
@@ -443,10 +447,10 @@ __device__ __forceinline__ void walk_neighbors(const Rec *__restrict__ mir, cons
   }
 }
 
-// Cell coordinates on the internal grid: cells of width h_eff + skin
-// (floor(x / width)), so that the 3x3 stencil of a cell list built at one
-// position still holds every pair within h_eff after each particle has moved
-// by up to skin / 2.  Periodic boxes clamp the upper edge.  Returns false
+// Cell coordinates on the internal grid: cells of height h_eff + skin and
+// width (h_eff + skin) / kx (floor(x / width)), so that the stencil (rows
+// -1..1, columns -kx..kx) of a cell list built at one position still holds
+// every pair within h_eff after each particle has moved by up to skin / 2.  Periodic boxes clamp the upper edge.  Returns false
 // outside the grid, also for a non-finite position (the range test is made
 // on the floating-point cell coordinate, before any conversion).
 __device__ __forceinline__ bool fluid_cell_xy(const InstDev &I, double x, double y, int &cx, int &cy) {
@@ -518,13 +522,13 @@ __device__ __forceinline__ void ref_cell_xy(const InstDev &I, double x, double y
 
 // Bookkeeping of a moved particle at slot s between cell builds.
 //   guard:  the particle may move at most skin / 2 from where the current
-//           cell list was built (xbuild); beyond that the 3x3 stencil could
+//           cell list was built (xbuild); beyond that the stencil could
 //           miss pairs, so the move is counted as an error;
 //   keys:   in the pass right before a build, the new cell key goes to
 //           kslot[s] (which held the key at the build); an escape is counted
 //           and clamped into the grid like the full build does; a move that
-//           leaves the 3x3 stencil of the old cell is queued on the stray
-//           list.
+//           leaves the stencil of the old cell (more than kx columns or one
+//           row) is queued on the stray list.
 // A non-finite position fails both checks.  In isolation mode (retired
 // non-null) such a move retires the particle's instance instead of raising
 // an error: the particle keeps its cell key, and from the next pass on the
@@ -606,19 +610,18 @@ __device__ __forceinline__ void record_move(const MoveOut &M, const InstDev &I, 
   if (!M.keys) return;
   const int nx = I.nx, ny = I.ny;
   const int oc = M.kslot[s] - I.cell_off;
+  // column and row distance between the old and the new cell (across the
+  // edges of a periodic box the shorter way round)
   int ddx = cx - oc % nx, ddy = cy - oc / nx;
+  ddx = ddx < 0 ? -ddx : ddx;
+  ddy = ddy < 0 ? -ddy : ddy;
   if (I.periodic) {
-    ddx = ddx < 0 ? ddx + nx : ddx;
-    ddy = ddy < 0 ? ddy + ny : ddy;
-    ddx = ddx == nx - 1 ? 1 : ddx;
-    ddy = ddy == ny - 1 ? 1 : ddy;
-  } else {
-    ddx = ddx < 0 ? -ddx : ddx;
-    ddy = ddy < 0 ? -ddy : ddy;
+    ddx = min(ddx, nx - ddx);
+    ddy = min(ddy, ny - ddy);
   }
   const int key = I.cell_off + cx + cy * nx;
   M.kslot[s] = key;
-  if (ddx > 1 || ddy > 1) {
+  if (ddx > I.kx || ddy > 1) {
     const int k = atomicAdd(M.nstray, 1);
     if (k < kStrayCap) {
       M.strays[2 * k] = s;

@@ -133,7 +133,7 @@ __global__ void k_download_rhs(int n_int, const int32_t *__restrict__ phys_of, R
 // cell build).
 __global__ void k_upload(int n, const double *__restrict__ stage, const int32_t *__restrict__ ptype,
                          const int32_t *__restrict__ inst_of, const InstDev *__restrict__ inst, Rec *rec,
-                         int32_t *info, int has_tem) {
+                         float4 *nbf, int32_t *info, int has_tem) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Rec r;
@@ -146,7 +146,7 @@ __global__ void k_upload(int n, const double *__restrict__ stage, const int32_t 
   r.tem = has_tem ? stage[6 * (size_t)n + i] : 0.0;
   const int ii = inst_of[i];
   r.vol = inst[ii].sp_mass / r.rho;
-  rec[i] = r;
+  store_rec(rec, nbf, i, r);
   info[i] = make_info(ptype[i], ii);
 }
 
@@ -187,7 +187,7 @@ void launch_upload(hipStream_t s, Device &d, const double *stage, const int32_t 
                    int has_tem) {
   const int n = d.n_total;
   hipLaunchKernelGGL(k_upload, dim3(blocks(n, 256)), dim3(256), 0, s, n, stage, ptype, inst_of, d.inst,
-                     d.mir[0], d.info, has_tem);
+                     d.mir[0], d.nbf[0], d.info, has_tem);
 }
 
 }  // namespace lssph

@@ -153,6 +153,11 @@ module lssph_binding
             type(c_ptr), value :: h
             real(c_double), value :: skin
         end function
+        integer(c_int32_t) function lssph_set_cell_split(h, columns) bind(C, name="lssph_set_cell_split")
+            import :: c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: columns
+        end function
         integer(c_int32_t) function lssph_set_scan(h, staged) bind(C, name="lssph_set_scan")
             import :: c_int32_t, c_ptr
             type(c_ptr), value :: h

@@ -53,7 +53,7 @@ EXPORTS = ("lssph_create", "lssph_destroy", "lssph_last_error", "lssph_upload", 
            "lssph_neighbors", "lssph_stats", "lssph_reset_stats", "lssph_synchronize",
            "lssph_set_tuning", "lssph_set_skin", "lssph_set_scan", "lssph_stream", "lssph_profile", "lssph_kernel_times",
            "lssph_pair_lists", "lssph_probe", "lssph_set_isolation", "lssph_instance_status",
-           "lssph_probe_static", "lssph_set_wave_particles")
+           "lssph_probe_static", "lssph_set_wave_particles", "lssph_set_cell_split")
 
 
 def load_library(path: Path | None = None):
@@ -96,6 +96,7 @@ def load_library(path: Path | None = None):
     lib.lssph_probe_static.argtypes = [vp, i32, i32, i32, i32, _D, _D, _D]
     lib.lssph_set_isolation.argtypes = [vp, i32]
     lib.lssph_set_wave_particles.argtypes = [vp, i32]
+    lib.lssph_set_cell_split.argtypes = [vp, i32]
     lib.lssph_instance_status.argtypes = [vp, _I, _I]
     for name in EXPORTS:
         if name not in ("lssph_last_error", "lssph_stream"):
@@ -124,7 +125,7 @@ class Simulation:
     instances, then their wall blocks."""
 
     def __init__(self, params, num_int: Sequence[int], num_ext: Sequence[int], precision: int | None = None,
-                 skin: float | None = None):
+                 skin: float | None = None, cell_split: int | None = None):
         self.lib = load_library()
         plist = params if isinstance(params, (list, tuple)) else [params]
         cparams = [p.to_c(precision) if isinstance(p, Param) else p for p in plist]
@@ -152,6 +153,8 @@ class Simulation:
         self.h = h
         if skin is not None:
             self.set_skin(skin)
+        if cell_split is not None:
+            self.set_cell_split(cell_split)
 
     # -- lifecycle ------------------------------------------------------------
     def close(self):
@@ -337,6 +340,11 @@ class Simulation:
         """Cell-list skin as a fraction of h_eff (0: rebuild after every pass)."""
         self._check(self.lib.lssph_set_skin(self.h, ctypes.c_double(skin)))
 
+    def set_cell_split(self, columns: int):
+        """Cell columns per stencil reach in x (1-4): the stencil is then 2 columns + 1
+        cells wide and tests fewer candidates; the neighbor sets do not change."""
+        self._check(self.lib.lssph_set_cell_split(self.h, int(columns)))
+
     def set_wave_particles(self, ppw: int):
         """Particles per wavefront of the SPH_MODEL 6 kernels (1, 2; 0: default)."""
         self._check(self.lib.lssph_set_wave_particles(self.h, int(ppw)))
@@ -390,9 +398,9 @@ def split_states(arrs: dict, num_int: Sequence[int], num_ext: Sequence[int]) -> 
 
 
 def simulation_from_state(param: Param, st: State, precision: int | None = None,
-                          skin: float | None = None) -> Simulation:
+                          skin: float | None = None, cell_split: int | None = None) -> Simulation:
     """Creates a handle for one box, uploads st and builds the cell list."""
-    sim = Simulation(param, [st.num_int], [st.num_ext], precision=precision, skin=skin)
+    sim = Simulation(param, [st.num_int], [st.num_ext], precision=precision, skin=skin, cell_split=cell_split)
     sim.upload(st)
     sim.init_cell()
     return sim

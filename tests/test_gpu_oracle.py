@@ -262,6 +262,7 @@ def test_misuse_returns_error_codes_and_leaves_the_handle_working():
 
     with Simulation(param, [st.num_int], [st.num_ext]) as s, simulation_from_state(param, st) as ref:
         assert code(s.step, 1) == -3  # before lssph_upload
+        assert code(s.set_cell_split, 0) == -1 and code(s.set_cell_split, 5) == -1
         assert code(s.calculate_RHS) == -3
         assert code(s.download) == -3
         s.upload(st)
@@ -270,6 +271,7 @@ def test_misuse_returns_error_codes_and_leaves_the_handle_working():
         assert code(s.RK2) == -1  # the handle was created for RK4
         assert code(s.step, -1) == -1
         assert code(s.set_skin, 0.1) == -3  # the cell grid is fixed at upload
+        assert code(s.set_cell_split, 3) == -3
         assert s.lib.lssph_set_scan(s.h, 2) == -1
         assert code(s.set_tuning, 32) == -1
         bad = {k: getattr(st, k).copy() for k in fields}
@@ -318,7 +320,7 @@ def test_mixed_precision_close_to_fp64_other_kernels(kernel_type, box):
     """The mixed path's single-precision kernel shapes (cubic spline,
     quintic spline, Wendland C4, C6; Wendland C2 is covered above) against
     the FP64 path, which the reference's fixtures pin for each.  After 10
-    steps the measured norm-wise differences are at most 4.7e-8 (u, v) and
+    steps the measured norm-wise differences are at most 4.9e-8 (u, v) and
     1.2e-8 (rho) over the eight cases; the bounds leave about 20x room, while
     a wrong shape, derivative or normalisation constant is off by far more."""
     if box == "periodic":
@@ -708,3 +710,67 @@ def test_wave_particles_rejects_bad_values():
         for bad in (3, 4, -1):  # 1 or 2 (0: the default)
             with pytest.raises(LssphError):
                 s.set_wave_particles(bad)
+
+
+@pytest.mark.parametrize("split", [1, 3, 4])
+@pytest.mark.parametrize("case", ["periodic", "smallest", "cavity"])
+def test_cell_split_matches_oracle(split, case):
+    """Internal cells 1 / split of the stencil reach wide in x (the default
+    is 2, which every other test runs): the stencil covers 2 split + 1
+    columns, wraps across periodic edges on split columns per side, and the
+    incremental builds move particles between narrower cells.  Neighbor
+    lists in summation order, the first RHS, and the states after steps
+    against the oracle; a full rebuild gives the same lists."""
+    if case == "periodic":
+        param, st = workloads.taylor_green_periodic(n=40, re=100.0, rk=4, jitter=0.3, seed=41)
+    elif case == "smallest":  # 3 cell rows, 3 split columns: every stencil wraps
+        param, st = workloads.taylor_green_periodic(n=10, re=100.0, rk=4, jitter=0.2, seed=42)
+    else:
+        param, st = workloads.cavity(n=24, re=100.0, rk=2)
+    print(f"[gpu] cell split {split}, {case}", flush=True)
+    o = OracleSim(param.to_c(), st.num_int, st.num_ext)
+    o.init(st)
+    with simulation_from_state(param, st, cell_split=split) as s:
+        o.ghost()
+        s.ghost()
+        co, lo = o.neighbors()
+        cs, ls = s.neighbors()
+        assert np.array_equal(co, cs) and np.array_equal(lo, ls)
+        r_o, r_s = o.calculate_rhs(), s.calculate_RHS()
+        assert max(_cmp(r_s, r_o, ("u", "v", "rho")).values()) <= TOL
+        for _ in range(4):
+            o.step(1)
+            s.step(1)
+        co, lo = o.neighbors()
+        cs, ls = s.neighbors()
+        assert np.array_equal(co, cs) and np.array_equal(lo, ls)
+        s.update_cell()
+        cf, lf = s.neighbors()
+        assert np.array_equal(cf, cs) and np.array_equal(lf, ls)
+        e = _cmp(s.download(), o.get_state(), ("x", "y", "u", "v", "rho", "pre"))
+        assert max(e.values()) <= TOL, e
+
+
+@pytest.mark.parametrize("split", [1, 3])
+def test_cell_split_far_moves_and_wraps(split):
+    """Particles carried one to three cells per stage with the cells rebuilt
+    after every pass (skin 0), across both periodic edges: with narrower
+    cells more of them leave the stencil of their old cell and go through
+    the stray list; states and neighbor lists stay the oracle's."""
+    param, st = workloads.taylor_green_periodic(n=24, re=100.0, rk=4, jitter=0.3, seed=43)
+    cw = 1.0 / int(1.0 / param.h_eff)
+    speed = 2.0 * cw / param.dt
+    st.u[:] = speed
+    st.v[:] = -0.6 * speed
+    o = OracleSim(param.to_c(), st.num_int, st.num_ext)
+    o.init(st)
+    print(f"[gpu] cell split {split}, far moves", flush=True)
+    with simulation_from_state(param, st, skin=0.0, cell_split=split) as s:
+        for _ in range(2):
+            o.step(1)
+            s.step(1)
+            co, lo = o.neighbors()
+            cs, ls = s.neighbors()
+            assert np.array_equal(co, cs) and np.array_equal(lo, ls)
+        e = _cmp(s.download(), o.get_state(), ("x", "y", "u", "v", "rho", "pre"))
+        assert max(e.values()) <= TOL, e

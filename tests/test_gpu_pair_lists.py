@@ -45,10 +45,10 @@ def _compare(sim, ref_sets):
     return {int(p): int(np.sum(path == p)) for p in np.unique(path)}
 
 
-def _run(param, st, staged=True, list_len=0, steps=0):
+def _run(param, st, staged=True, list_len=0, steps=0, split=None):
     print(f"[gpu] pair lists n_int={st.num_int} n_ext={st.num_ext} periodic={param.cfg.periodic} "
-          f"staged={staged} list_len={list_len} steps={steps}", flush=True)
-    with simulation_from_state(param, st, precision=1) as s:
+          f"staged={staged} list_len={list_len} steps={steps} split={split}", flush=True)
+    with simulation_from_state(param, st, precision=1, cell_split=split) as s:
         s.set_scan(staged)
         if list_len:
             s.set_tuning(list_len=list_len)
@@ -123,3 +123,16 @@ def test_ensemble_lists():
         sim.ghost()
         paths = _compare(sim, ref)
     assert paths.get(0, 0) > 0
+
+
+@pytest.mark.parametrize("split", [1, 3, 4])
+def test_lists_with_other_cell_splits(split):
+    """The default cells are half the stencil reach wide in x; narrower and
+    square cells give the same sets on every scan path, periodic edges,
+    walls and incremental builds included."""
+    param, st = workloads.taylor_green_periodic(n=64, re=1000.0, rk=4, jitter=0.2, seed=28)
+    paths = _run(param, st, split=split)
+    assert paths.get(2, 0) > 0 and paths.get(0, 0) > 0
+    _run(param, st, list_len=30, split=split, steps=2)
+    param, st = workloads.cavity(n=32, re=400.0, jitter=0.1, seed=29)
+    _run(param, st, split=split, steps=2)

@@ -14,8 +14,9 @@ class FakeSim:
 
     log = []
 
-    def __init__(self, skin):
+    def __init__(self, skin, cell_split=0):
         self.skin = skin
+        self.cell_split = cell_split
         self.block, self.list_len = 128, 0
         self.ppw = 0
         self.clock = 0.0
@@ -36,7 +37,8 @@ class FakeSim:
         if self.skin == 0.02:
             raise LssphError(-4, "a particle moved farther than half the cell-list skin")
         cost = {64: 1.3, 128: 1.0, 256: 1.1}[self.block] * (1.0 + abs(self.skin - 0.08)) * (
-            1.05 if self.list_len == 64 else 1.0) * (0.8 if self.ppw == 2 else 1.0)
+            1.05 if self.list_len == 64 else 1.0) * (0.8 if self.ppw == 2 else 1.0) * (
+            {0: 1.0, 1: 1.1, 3: 0.9}[self.cell_split])
         self.clock += cost * n
 
     def synchronize(self):
@@ -50,8 +52,8 @@ class FakeSim:
 def fake_clock(monkeypatch):
     sims = []
 
-    def make(skin):
-        s = FakeSim(skin)
+    def make(skin, *cell_split):
+        s = FakeSim(skin, *cell_split)
         sims.append(s)
         return s
 
@@ -99,3 +101,14 @@ def test_autotune_covers_particles_per_wavefront(fake_clock):
     bad = [s for s in r.table if s.error]
     assert [s.wave_particles for s in bad] == [4]
     assert r.as_dict()["best"]["wave_particles"] == 2
+
+
+def test_autotune_covers_cell_splits(fake_clock):
+    """The cell columns per stencil reach are fixed before the upload, like
+    the skin: each split gets a fresh handle, made with make_sim(skin) for
+    the library's default (0) and make_sim(skin, split) otherwise."""
+    r = tune.autotune(fake_clock, steps=2, blocks=(128,), skins=(0.08,), splits=(0, 1, 3))
+    assert [s.cell_split for s in r.table] == [0, 1, 3]
+    assert r.best.cell_split == 3 and r.best.ms_per_step == pytest.approx(0.9)
+    assert r.default.cell_split == 0 and r.default.ms_per_step == pytest.approx(1.0)
+    assert r.as_dict()["best"]["cell_split"] == 3
