@@ -32,7 +32,10 @@ extern "C" {
 #define LSSPH_E_ARG -1       /* invalid argument or unsupported option        */
 #define LSSPH_E_HIP -2       /* HIP runtime error (message has the HIP text)  */
 #define LSSPH_E_STATE -3     /* call out of order (e.g. step before upload)   */
-#define LSSPH_E_DOMAIN -4    /* a fluid particle left the cell grid           */
+#define LSSPH_E_DOMAIN -4    /* a fluid particle left the cell grid, or moved
+                                farther than the skin allows between builds;
+                                its instance stops advancing at that pass
+                                until the next lssph_upload                  */
 #define LSSPH_E_NOMEM -5     /* device allocation failed                      */
 #define LSSPH_E_CAPACITY -6  /* caller buffer too small                       */
 
@@ -240,7 +243,9 @@ int32_t lssph_probe_static(lssph_handle_t h, int32_t inst, int32_t field,
    the handle.  From the next pass on, the pair kernels and ghost walls leave
    that instance's records as they are; the other instances run on, and
    their results are the same as without the retired one.  0 (the default):
-   such a move is an LSSPH_E_DOMAIN error of the handle.  Turning it on
+   such a move is an LSSPH_E_DOMAIN error of the handle, reported by the next
+   call that returns host data, and the instance is frozen in the same way
+   (a diverged run stops costing time) until the next lssph_upload.  Turning it on
    clears every instance's status and restarts the step count; turning it
    off leaves instances already retired frozen (lssph_upload revives them
    with new data).  A too small skin (lssph_set_skin) for an instance's

@@ -65,8 +65,9 @@ __device__ __forceinline__ Run lane_runs(int key) {
 // Keys of every particle and per-cell counts, over the slots of the current
 // order (positions from the records in that order).  Fluid keys follow the
 // particle positions; wall keys are fixed at upload (fixed_key_mode = 1
-// computes them).  Escaped particles are counted and clamped into the grid
-// so that memory stays in bounds.
+// computes them).  Escaped particles (outside the grid, or at a non-finite
+// position) are counted as an error and spread over their instance's cells
+// by slot, so that memory stays in bounds and no cell collects them.
 __global__ void __launch_bounds__(256) k_bin_count(int n, const Rec *__restrict__ rec,
                                                    const int32_t *__restrict__ items,
                                                    const int32_t *__restrict__ info,
@@ -84,7 +85,7 @@ __global__ void __launch_bounds__(256) k_bin_count(int n, const Rec *__restrict_
       int c = info_ptype(inf) == 0 ? fluid_cell(I, xy.x, xy.y) : fixed_cell(I, xy.x, xy.y);
       if (c < 0) {
         atomicAdd(escaped, 1);
-        c = 0;
+        c = t % (I.nx * I.ny);
       }
       k = I.cell_off + c;
       key[p] = k;

@@ -532,7 +532,13 @@ __device__ __forceinline__ void ref_cell_xy(const InstDev &I, double x, double y
 // A non-finite position fails both checks.  In isolation mode (retired
 // non-null) such a move retires the particle's instance instead of raising
 // an error: the particle keeps its cell key, and from the next pass on the
-// kernels leave the instance's records as they are.
+// kernels leave the instance's records as they are.  Otherwise the move is
+// counted as the handle's error, and the particle and its instance are
+// frozen the same way until the next upload.  A diverging run would
+// otherwise step on until the next synchronising call reports the error,
+// binning every escaped particle into one cell: that cell would grow by up
+// to a stray list per build, and its members' scans and its rebuild would
+// grow with the square of its size.
 struct MoveOut {
   int32_t *kslot;
   int32_t *strays;
@@ -593,19 +599,19 @@ __device__ __forceinline__ void record_move(const MoveOut &M, const InstDev &I, 
     if (!(dx * dx + dy * dy <= I.skin_half_sq)) reason = LSSPH_RETIRED_SKIN;
   }
   int cx = 0, cy = 0;
-  if (M.keys && !fluid_cell_xy(I, x, y, cx, cy)) {
-    reason = LSSPH_RETIRED_ESCAPED;
-    cx = cy = 0;
-  }
+  if (M.keys && !fluid_cell_xy(I, x, y, cx, cy)) reason = LSSPH_RETIRED_ESCAPED;
   if (reason != 0) {
+    // the particle keeps its cell key either way
     if (M.retired) {
-      retire_instance(M, s, reason);  // the particle keeps its cell key
-      return;
+      retire_instance(M, s, reason);
+    } else {
+      if (reason == LSSPH_RETIRED_SKIN)
+        M.counters[kSkinExceeded] = 1;
+      else
+        atomicAdd(&M.counters[kEscaped], 1);
+      M.inst_rw[info_inst(M.sinfo[s])].retired = 1;  // frozen until the next upload
     }
-    if (reason == LSSPH_RETIRED_SKIN)
-      M.counters[kSkinExceeded] = 1;
-    else
-      atomicAdd(&M.counters[kEscaped], 1);  // clamped into the grid like the full build does
+    return;
   }
   if (!M.keys) return;
   const int nx = I.nx, ny = I.ny;

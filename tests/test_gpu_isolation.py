@@ -67,6 +67,43 @@ def test_without_isolation_a_diverged_instance_fails_the_handle():
         assert e.value.code == -4
 
 
+def test_without_isolation_a_nan_instance_is_reported_and_stops_advancing():
+    """Without isolation a non-finite position is an LSSPH_E_DOMAIN error of
+    the handle, reported by the next synchronising call, and the instance
+    stops advancing at the pass that made it: its particles keep their cells
+    instead of being binned, step after step, into one cell whose scans and
+    rebuilds would grow with the square of its size.  Steps after the error
+    cost what steps of a healthy handle cost."""
+    import time
+    params, states = _ensemble(_nan)
+    ni = [s.num_int for s in states]
+    ne = [s.num_ext for s in states]
+    with Simulation(params, ni, ne, precision=1) as sim:
+        sim.upload(states)
+        sim.init_cell()
+        sim.eager_step(2)
+        sim.stats()  # a synchronising call that does not check the domain counters
+        t0 = time.perf_counter()
+        sim.eager_step(20)
+        sim.stats()
+        frozen_ms = (time.perf_counter() - t0) * 1e3 / 20
+        with pytest.raises(LssphError) as e:
+            sim.download()
+        assert e.value.code == -4
+    healthy, hstates = _ensemble(lambda st: None)
+    with Simulation(healthy, ni, ne, precision=1) as sim:
+        sim.upload(hstates)
+        sim.init_cell()
+        sim.eager_step(2)
+        sim.stats()
+        t0 = time.perf_counter()
+        sim.eager_step(20)
+        sim.stats()
+        healthy_ms = (time.perf_counter() - t0) * 1e3 / 20
+    print(f"[gpu] ms per eager step: after the error {frozen_ms:.3f}, healthy {healthy_ms:.3f}", flush=True)
+    assert frozen_ms <= 2.0 * healthy_ms + 1.0
+
+
 def test_isolation_restarts_its_count_and_a_healthy_ensemble_retires_nothing():
     params, states = _ensemble(lambda st: None)
     ni = [s.num_int for s in states]
