@@ -128,6 +128,7 @@ contains
         if (c_associated(dev)) call lssph_check(lssph_destroy(dev), "lssph_destroy")
         call lssph_check(lssph_create(p, 1_c_int32_t, n_int, n_ext, dev), "lssph_create")
         call lssph_check(lssph_set_skin(dev, param%cell_skin), "lssph_set_skin")
+        call lssph_check(lssph_set_cell_split(dev, int(param%cell_split, c_int32_t)), "lssph_set_cell_split")
         call lssph_check(lssph_upload(dev, SP%ptype, SP%x, SP%y, SP%u, SP%v, SP%rho, SP%pre, SP%tem), &
                          "lssph_upload")
         call lssph_check(lssph_init_cell(dev), "lssph_init_cell")

@@ -41,7 +41,7 @@ module global_types
         double precision   :: dt, dt_CFL, dt_vis, dt_th
         double precision   :: dt_CFL_relax, dt_vis_relax
         ! options of this host
-        integer            :: rk, periodic, precision
+        integer            :: rk, periodic, precision, cell_split
         double precision   :: diff_coef, diff_source, cond_max, cell_skin
     end type Param_type
 
@@ -176,6 +176,7 @@ contains
         param%diff_source    = DIFF_SOURCE
         param%cond_max       = COND_MAX
         param%cell_skin      = CELL_SKIN
+        param%cell_split     = CELL_SPLIT
 
         if (trim(adjustl(param%read_name)) == "new") then
             param%total_step = param%end_step

@@ -27,6 +27,9 @@
 #ifndef CELL_SKIN
 #define CELL_SKIN 0.08d0    /* cell-list skin / h_eff; 0: a cell build after every pass */
 #endif
+#ifndef CELL_SPLIT
+#define CELL_SPLIT 2        /* cell columns per stencil reach in x (1-4) */
+#endif
 #ifndef USE_GRAPH
 #define USE_GRAPH 1         /* 1: steps replayed from a captured HIP graph */
 #endif
