@@ -316,10 +316,12 @@ int32_t lssph_set_cell_split(lssph_handle_t h, int32_t columns);
 int32_t lssph_set_scan(lssph_handle_t h, int32_t staged);
 
 /* Particles per wavefront of the SPH_MODEL 6 kernels (q = 20), a tuning
-   knob with no reference counterpart: a group of 64 / ppw lanes gathers,
-   sums and factors one particle's 20 x 20 system.  1 or 2; 0 restores the
-   default (2).  Both values give the same results bit for bit; the other
-   models (one particle per lane) ignore it. */
+   knob with no reference counterpart.  1 or 2: a group of 64 / ppw lanes
+   gathers, sums and factors one particle's 20 x 20 system in on-chip
+   memory; 3: a group of 20 lanes, one matrix row per lane, factors it with
+   the rows in registers.  0 restores the default (3).  All values give the
+   same results bit for bit; the other models (one particle per lane) ignore
+   it. */
 int32_t lssph_set_wave_particles(lssph_handle_t h, int32_t ppw);
 
 /* The HIP stream (hipStream_t) the handle launches on, for event timing. */

@@ -1111,7 +1111,7 @@ int32_t lssph_instance_status(lssph_handle_t h, int32_t *retired, int32_t *reaso
 
 int32_t lssph_set_wave_particles(lssph_handle_t h, int32_t ppw) {
   if (!h) return fail(nullptr, LSSPH_E_ARG, "null handle");
-  if (!(ppw == 0 || ppw == 1 || ppw == 2)) return fail(h, LSSPH_E_ARG, "lssph_set_wave_particles: 0, 1 or 2");
+  if (!(ppw >= 0 && ppw <= 3)) return fail(h, LSSPH_E_ARG, "lssph_set_wave_particles: 0, 1, 2 or 3");
   h->d.wave_ppw = ppw;
   drop_graphs(h);
   return 0;

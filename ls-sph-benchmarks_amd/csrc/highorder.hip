@@ -3,8 +3,10 @@
 // same basis (shift_interpolate_mod.f90:45-52, 153-185, 255-298), FP64.
 //
 // A 20 x 20 moment matrix (210 sums plus 20 per field) does not fit one
-// thread's registers, so a group of GS lanes (32 or 64: 2 or 1 particles per
-// wavefront, GS >= q) handles one particle:
+// thread's registers, so a group of lanes handles one particle.  Two designs
+// share the neighbor search and give the same results bit for bit: the row
+// groups below (20 lanes, 3 particles per wavefront, the default), and lane
+// groups of GS lanes (32 or 64: 2 or 1 particles per wavefront, GS >= q):
 //   1. the group tests the stencil's candidates GS at a time and compacts the
 //      accepted ones into an LDS list with a ballot;
 //   2. a rank sort puts the list into the reference's summation order
@@ -36,6 +38,7 @@ constexpr int kWavesPB = 2;  // wavefronts per block
 // Lanes of one particle's group: GS consecutive lanes of a wavefront.
 template <int GS>
 struct Group {
+  static constexpr int size = GS;
   static constexpr int G = 64 / GS;  // particles per wavefront
   int lane;  // lane in the wavefront
   int g;     // group in the wavefront
@@ -189,13 +192,16 @@ struct WaveLds {
 };
 
 // Neighbors of the particle at slot s (record me) within h_eff, in the
-// reference's summation order, into W.sorted.  Returns the count; a count
-// above kWaveList is flagged and only the first kWaveList are kept.
-template <int GS, class WL>
+// reference's summation order, into sorted[] (list[] and key[] are work
+// space).  Returns the count; a count above kWaveList is flagged and only the
+// first kWaveList are kept.  GR: the lanes of the particle's group (size,
+// mask, lane, gl).
+template <class GR>
 __device__ int group_neighbors(const Rec *__restrict__ mir, const int32_t *__restrict__ cstart,
                                const int32_t *__restrict__ sinfo, const int32_t *__restrict__ items,
                                const int32_t *__restrict__ orig_of, int32_t *counters, const InstDev &I, int s,
-                               const Rec &me, WL &W, const Group<GS> &G) {
+                               const Rec &me, int32_t *list, uint32_t *key, int32_t *sorted, const GR &G) {
+  constexpr int GS = GR::size;
   const int lc = local_cell(I, me.x, me.y);
   const unsigned long long below = G.mask & ((1ull << G.lane) - 1ull);
   int cnt = 0;
@@ -213,7 +219,7 @@ __device__ int group_neighbors(const Rec *__restrict__ mir, const int32_t *__res
       }
       const unsigned long long m = __ballot(ok) & G.mask;
       const int pos = cnt + __popcll(m & below);
-      if (ok && pos < kWaveList) W.list[pos] = t | (code << kSlotBits);
+      if (ok && pos < kWaveList) list[pos] = t | (code << kSlotBits);
       cnt += __popcll(m);
     }
   });
@@ -225,7 +231,7 @@ __device__ int group_neighbors(const Rec *__restrict__ mir, const int32_t *__res
   int rcx, rcy;
   ref_cell_xy(I, me.x, me.y, false, rcx, rcy);
   for (int e = G.gl; e < cnt; e += GS) {
-    const int t = W.list[e] & kSlotMask;
+    const int t = list[e] & kSlotMask;
     const double2 p = load_xy(mir, t);
     int jx, jy;
     ref_cell_xy(I, p.x, p.y, info_ptype(sinfo[t]) != 0, jx, jy);
@@ -234,15 +240,15 @@ __device__ int group_neighbors(const Rec *__restrict__ mir, const int32_t *__res
       dx = dx > 1 ? dx - I.rnx : (dx < -1 ? dx + I.rnx : dx);
       dy = dy > 1 ? dy - I.rny : (dy < -1 ? dy + I.rny : dy);
     }
-    W.key[e] = ((uint32_t)((dy + 1) * 3 + (dx + 1)) << kSlotBits) | (uint32_t)orig_of[items[t]];
+    key[e] = ((uint32_t)((dy + 1) * 3 + (dx + 1)) << kSlotBits) | (uint32_t)orig_of[items[t]];
   }
   wave_sync();
   // rank sort (the keys are distinct)
   for (int e = G.gl; e < cnt; e += GS) {
-    const uint32_t k = W.key[e];
+    const uint32_t k = key[e];
     int rank = 0;
-    for (int j = 0; j < cnt; ++j) rank += (W.key[j] < k) ? 1 : 0;
-    W.sorted[rank] = W.list[e];
+    for (int j = 0; j < cnt; ++j) rank += (key[j] < k) ? 1 : 0;
+    sorted[rank] = list[e];
   }
   wave_sync();
   return cnt;
@@ -382,6 +388,51 @@ __device__ __forceinline__ void inv_h_powers(const InstDev &I, double (&ih)[5]) 
   for (int k = 2; k < 5; ++k) ih[k] = ih[k - 1] / I.h;
 }
 
+// Terms of list entry e in the LS type-A sums of the particle me: the basis a,
+// V_j W and the field differences (LSSPH_A_mod.f90:130-183).
+template <int Q, int CLS>
+__device__ __forceinline__ void lsa_terms(const LsaArgs &A, const InstDev &I, const Rec &me, int e, int kt,
+                                          const double (&ih)[5], double (&a)[Q], double &vw,
+                                          double (&df)[Layout<CLS, true>::NF]) {
+  const int t = e & kSlotMask, code = e >> kSlotBits;
+  const Rec nb = A.mir[t];
+  const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
+  const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
+  const double r = sqrt(sq_dist(dx, dy));
+  vw = nb.vol * ref_w(kt, r, I.h);
+  basis_hq<Q>(dx, dy, ih, a);
+  field_diffs<CLS, true>(nb, me, df);
+}
+
+// Terms of list entry e in the shifting of the particle me: the basis a,
+// V_j W, the differences of u, v, rho (, tem) and the neighbor's two shift
+// terms (shift_interpolate_mod.f90:130-201).
+template <int Q, int NF>
+__device__ __forceinline__ void pst_terms(const PstArgs &A, const InstDev &I, const Rec &me, int e, int kt,
+                                          const double (&ih)[5], double (&a)[Q], double &vw, double (&df)[NF],
+                                          double &shx, double &shy) {
+  const int t = e & kSlotMask, code = e >> kSlotBits;
+  const Rec nb = A.mir[t];
+  const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
+  const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
+  const double r = sqrt(sq_dist(dx, dy));
+  const double wv = ref_w(kt, r, I.h);
+  const double dwdr = ref_dw(kt, r, I.h);
+  const double vj = nb.vol;
+  vw = vj * wv;
+  const double dwx = dwdr * (-dx) / r * vj;  // :142-143
+  const double dwy = dwdr * (-dy) / r * vj;
+  const double wr = wv / I.w_ave;
+  const double fac = 1.0 + 0.2 * ((wr * wr) * (wr * wr));
+  basis_hq<Q>(dx, dy, ih, a);
+  df[0] = nb.u - me.u;
+  df[1] = nb.v - me.v;
+  df[2] = nb.rho - me.rho;
+  if constexpr (NF == 4) df[3] = nb.tem - me.tem;
+  shx = fac * dwx;
+  shy = fac * dwy;
+}
+
 // ---------------------------------------------------------------------------
 // LS type A (LSSPH_A_mod.f90:26-271) with q = 14, 20 and the stage update.
 // The field columns are those of the q = 5 kernels with rho always carried.
@@ -407,7 +458,8 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_lsa_wave(LsaArgs A) {
     return;
   }
   const Rec me = A.mir[s];
-  const int n = group_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.counters, I, s, me, W, G);
+  const int n = group_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.counters, I, s, me, W.list, W.key,
+                                W.sorted, G);
   double ih[5];
   inv_h_powers(I, ih);
   const int kt = A.kt;
@@ -416,17 +468,8 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_lsa_wave(LsaArgs A) {
   for (int c0 = 0; c0 < n; c0 += kChunk) {
     const int nc = min(kChunk, n - c0);
     if (G.gl < nc) {
-      const int e = W.sorted[c0 + G.gl];
-      const int t = e & kSlotMask, code = e >> kSlotBits;
-      const Rec nb = A.mir[t];
-      const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
-      const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
-      const double r = sqrt(sq_dist(dx, dy));
-      const double vw = nb.vol * ref_w(kt, r, I.h);
-      double a[Q];
-      basis_hq<Q>(dx, dy, ih, a);
-      double df[NF];
-      field_diffs<CLS, true>(nb, me, df);
+      double a[Q], vw, df[NF];
+      lsa_terms<Q, CLS>(A, I, me, W.sorted[c0 + G.gl], kt, ih, a, vw, df);
       double *row = W.u.rows[G.gl];
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
@@ -482,7 +525,8 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
     return;
   }
   const Rec me = A.mir[s];
-  const int n = group_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.mv.counters, I, s, me, W, G);
+  const int n = group_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.mv.counters, I, s, me, W.list,
+                                W.key, W.sorted, G);
   double ih[5];
   inv_h_powers(I, ih);
   const int kt = A.kt;
@@ -492,34 +536,18 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
   for (int c0 = 0; c0 < n; c0 += kChunk) {
     const int nc = min(kChunk, n - c0);
     if (G.gl < nc) {
-      const int e = W.sorted[c0 + G.gl];
-      const int t = e & kSlotMask, code = e >> kSlotBits;
-      const Rec nb = A.mir[t];
-      const double dx = (nb.x + shift_of(code & 3, I.len_x)) - me.x;
-      const double dy = (nb.y + shift_of(code >> 2, I.len_y)) - me.y;
-      const double r = sqrt(sq_dist(dx, dy));
-      const double wv = ref_w(kt, r, I.h);
-      const double dwdr = ref_dw(kt, r, I.h);
-      const double vj = nb.vol;
-      const double vw = vj * wv;
-      const double dwx = dwdr * (-dx) / r * vj;  // :142-143
-      const double dwy = dwdr * (-dy) / r * vj;
-      const double wr = wv / I.w_ave;
-      const double fac = 1.0 + 0.2 * ((wr * wr) * (wr * wr));
-      double a[Q];
-      basis_hq<Q>(dx, dy, ih, a);
+      double a[Q], vw, df[NF], shx, shy;
+      pst_terms<Q, NF>(A, I, me, W.sorted[c0 + G.gl], kt, ih, a, vw, df, shx, shy);
       double *row = W.u.rows[G.gl];
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         row[i] = a[i];
         row[Q + i] = vw * a[i];
       }
-      row[2 * Q + 0] = nb.u - me.u;
-      row[2 * Q + 1] = nb.v - me.v;
-      row[2 * Q + 2] = nb.rho - me.rho;
-      if constexpr (T) row[2 * Q + 3] = nb.tem - me.tem;
-      row[2 * Q + NF] = fac * dwx;
-      row[2 * Q + NF + 1] = fac * dwy;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) row[2 * Q + f] = df[f];
+      row[2 * Q + NF] = shx;
+      row[2 * Q + NF + 1] = shy;
     }
     wave_sync();
     for (int k = 0; k < nc; ++k) {  // the shift sums in list order (:145-146)
@@ -573,6 +601,381 @@ __global__ void __launch_bounds__(64 * kWavesPB) k_pst_wave(PstArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// Row groups: Q lanes per particle, 64 / Q particles per wavefront (3 at
+// q = 20); lane i of a group owns row i of its particle's moment matrix.
+//   1. neighbor search and summation order as above (group_neighbors);
+//   2. in chunks of Q neighbors, lane k forms a, V_j W and the field
+//      differences of neighbor k in LDS; then lane i accumulates column i of
+//      the upper triangle, M(r, i) += (V_j W a_i) a_r for r <= i, and
+//      b(i, f) += (V_j W a_i) df_f, in list order: the operands and order of
+//      the lane-group kernels above, so the sums agree bit for bit;
+//   3. the group factors the symmetric matrix with partial pivoting without
+//      moving rows: each lane keeps its row in registers and the row's
+//      position; LDS holds a copy of every row by position and the pivot
+//      column's candidates, from which every lane takes the pivot (the first
+//      position of the largest |a|) and the pivot row; each elimination step
+//      also updates the lane's right-hand sides -- the forward substitution
+//      of lu_apply, in its order -- and writes the row back;
+//   4. lane f of the group runs the back substitution of field f over U in
+//      LDS.
+// Every operation of group_dgesv is done once, on the same operands and in
+// the same order, so both kernels give the same results bit for bit.
+// ---------------------------------------------------------------------------
+template <int Q>
+struct RowGroup {
+  static constexpr int size = Q;
+  static constexpr int G = 64 / Q;  // particles per wavefront
+  int lane;  // lane in the wavefront
+  int g;     // group in the wavefront (G: an idle lane)
+  int gl;    // lane in the group: the row it owns
+  unsigned long long mask;
+  __device__ __forceinline__ explicit RowGroup(int l) : lane(l), g(l / Q), gl(l % Q) {
+    mask = ((1ull << Q) - 1ull) << (g * Q);
+  }
+};
+
+// LDS of one particle's row group (4.8 KB at q = 20).
+template <int Q, int NF, int NX>
+struct alignas(16) RowLds {
+  static constexpr int RS = (Q + 1 + NF + NX + 1) & ~1;  // even: every row 16-B aligned
+  static constexpr int VW = Q, DF = Q + 1, XT = Q + 1 + NF;  // row offsets after a[0..Q)
+  int32_t sorted[kWaveList];
+  union {
+    struct {
+      int32_t list[kWaveList];
+      uint32_t key[kWaveList];
+    } scan;
+    double rows[Q][RS];  // chunk: a, V_j W, df, extra terms of neighbor k
+    struct {
+      double r[Q][RS];   // the row at position p: r[p][0..Q), right-hand sides r[p][Q..Q + NF)
+      double cand[Q];    // |r[p][j]| of the rows not yet pivots
+    } lu;
+  } u;
+};
+
+// (v, key) over the group's lanes: the largest v, ties to the smaller key;
+// every lane of the group gets the result.
+template <int Q>
+__device__ __forceinline__ void row_argmax(double &v, int &key, const RowGroup<Q> &G) {
+#pragma unroll
+  for (int off = 1; off < Q; off <<= 1) {
+    const double v2 = __shfl(v, G.lane + off, 64);
+    const int k2 = __shfl(key, G.lane + off, 64);
+    const bool take = (G.gl + off < Q) && ((v2 > v) || (v2 == v && k2 < key));
+    v = take ? v2 : v;
+    key = take ? k2 : key;
+  }
+  v = __shfl(v, G.g * Q, 64);
+  key = __shfl(key, G.g * Q, 64);
+}
+
+// Per-lane sums of a row group (step 2): acc[r] = M(r, i) for r <= i (the
+// entries above i are not used), bacc[f] = b(i, f).
+template <int Q, int NF>
+struct RowSums {
+  double acc[Q];
+  double bacc[NF];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int r = 0; r < Q; ++r) acc[r] = 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) bacc[f] = 0.0;
+  }
+  template <class RL>
+  __device__ __forceinline__ void add_chunk(const RL &W, int nc, int i) {
+    for (int k = 0; k < nc; ++k) {
+      const double *row = W.u.rows[k];
+      const double vwa = row[RL::VW] * row[i];  // V_j W a_i
+#pragma unroll
+      for (int r = 0; r < Q; ++r) acc[r] += vwa * row[r];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) bacc[f] += vwa * row[RL::DF + f];
+    }
+  }
+};
+
+// f(std::integral_constant<int, J>) for J = B .. E - 1, in order.
+template <int B, int E>
+struct StaticFor {
+  template <class F>
+  static __device__ __forceinline__ void run(F &&f) {
+    if constexpr (B < E) {
+      f(std::integral_constant<int, B>{});
+      StaticFor<B + 1, E>::run(f);
+    }
+  }
+};
+
+// First position of the largest value among c[LO..LO + N) (ties and NaNs keep
+// the earlier position), as a tree over the two halves.
+template <int LO, int N>
+struct FirstMax {
+  template <int M>
+  static __device__ __forceinline__ void run(const double (&c)[M], double &v, int &p) {
+    if constexpr (N == 1) {
+      v = c[LO];
+      p = LO;
+    } else {
+      constexpr int H = (N + 1) / 2;
+      double v1, v2;
+      int p1, p2;
+      FirstMax<LO, H>::run(c, v1, p1);
+      FirstMax<LO + H, N - H>::run(c, v2, p2);
+      const bool take = v2 > v1;
+      v = take ? v2 : v1;
+      p = take ? p2 : p1;
+    }
+  }
+};
+
+// DGESV on the sums of a row group (steps 3 and 4).  The group's rows are
+// kept by position in LDS (R[p] = the row at position p and its right-hand
+// sides, cand[p] = |R[p][j]| for the next pivot column): every lane reads
+// the candidates and the pivot row from there, and the lanes of the rows
+// that are not yet pivots write theirs back after each elimination step.
+// On return R[k][Q + f] holds the solution of field f -- or, when a pivot
+// is zero, the unsolved right-hand sides, as DGESV leaves them.
+template <int Q, int NF, class RL>
+__device__ __forceinline__ void row_dgesv(RL &W, const RowSums<Q, NF> &S, const RowGroup<Q> &G) {
+  auto &R = W.u.lu.r;
+  auto &cand = W.u.lu.cand;
+  const int i = G.gl;
+  // M(r, i), r <= i, is acc[r] of lane i: row i, column r and row r, column i
+#pragma unroll
+  for (int r = 0; r < Q; ++r)
+    if (r <= i) {
+      R[i][r] = S.acc[r];
+      R[r][i] = S.acc[r];
+    }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) R[i][Q + f] = S.bacc[f];
+  wave_sync();
+  double row[Q], b[NF];
+#pragma unroll
+  for (int r = 0; r < Q; ++r) row[r] = R[i][r];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) b[f] = S.bacc[f];
+  cand[i] = fabs(row[0]);
+  wave_sync();
+  int pos = i;        // the position of this lane's row
+  bool done = false;  // the row was a pivot row
+  bool singular = false;
+  StaticFor<0, Q>::run([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    // the pivot: the first position >= j of the largest |a(., j)|
+    double c[Q - j];
+#pragma unroll
+    for (int k = 0; k < Q - j; ++k) c[k] = cand[j + k];
+    double cmax;
+    int bk;
+    FirstMax<0, Q - j>::run(c, cmax, bk);
+    const int bp = j + bk;
+    double u[Q - j], ub[NF];
+#pragma unroll
+    for (int k = 0; k < Q - j; ++k) u[k] = R[bp][j + k];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) ub[f] = R[bp][Q + f];
+    wave_sync();  // the reads of R and cand come before this step's writes
+    const double pv = u[0];
+    singular = singular || pv == 0.0;
+    const bool active = !done;  // the row is at a position >= j
+    const bool piv = pos == bp;
+    pos = piv ? j : (pos == j ? bp : pos);
+    done = done || piv;
+    // the rows below the pivot: multiplier, trailing update, forward
+    // substitution (a zero pivot skips the division); the others add zeros
+    const double lq = pv != 0.0 ? row[j] / pv : row[j];
+    const double l = done ? 0.0 : lq;
+    row[j] = done ? row[j] : lq;
+#pragma unroll
+    for (int k = 1; k < Q - j; ++k) row[j + k] -= l * u[k];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) b[f] -= ub[f] * l;
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < Q - j; ++k) R[pos][j + k] = row[j + k];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) R[pos][Q + f] = b[f];
+      if constexpr (j + 1 < Q)
+        if (!done) cand[pos] = fabs(row[j + 1]);
+    }
+    wave_sync();
+  });
+  // back substitution, one field per lane (lu_apply)
+  if (!singular) {
+    if (i < NF) {
+      double x[Q];
+#pragma unroll
+      for (int p = 0; p < Q; ++p) x[p] = R[p][Q + i];
+#pragma unroll
+      for (int j = Q - 1; j >= 0; --j) {
+        x[j] = x[j] / R[j][j];
+#pragma unroll
+        for (int p = 0; p < j; ++p) x[p] -= x[j] * R[p][j];
+      }
+#pragma unroll
+      for (int p = 0; p < Q; ++p) R[p][Q + i] = x[p];
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) R[i][Q + f] = S.bacc[f];
+  }
+  wave_sync();
+}
+
+// LS type A with q = 20 on row groups (see above) and the stage update.
+template <int Q, int CLS>
+__global__ void __launch_bounds__(64) k_lsa_rows(LsaArgs A) {
+  using L = Layout<CLS, true>;
+  constexpr int NF = L::NF;
+  using RL = RowLds<Q, NF, 0>;
+  constexpr int NG = RowGroup<Q>::G;
+  __shared__ RL lds_all[NG];
+  const RowGroup<Q> G(threadIdx.x);
+  if (G.g >= NG) return;
+  const int s = blockIdx.x * NG + G.g;
+  if (s >= A.n) return;
+  RL &W = lds_all[G.g];
+  const int32_t inf = A.sinfo[s];
+  const InstDev &I = A.inst[info_inst(inf)];
+  if (info_ptype(inf) != 0 || I.retired) {
+    if (G.gl == 0 && A.mode != kRhsOnly) A.mout[s] = A.mir[s];
+    if (G.gl == 0 && A.spd) A.spd[s] = 0.0;
+    return;
+  }
+  const Rec me = A.mir[s];
+  const int n = group_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.counters, I, s, me,
+                                W.u.scan.list, W.u.scan.key, W.sorted, G);
+  double ih[5];
+  inv_h_powers(I, ih);
+  const int kt = A.kt;
+  RowSums<Q, NF> S;
+  S.zero();
+  for (int c0 = 0; c0 < n; c0 += Q) {
+    const int nc = min(Q, n - c0);
+    wave_sync();  // the previous chunk's (or the sort's) reads come first
+    if (G.gl < nc) {
+      double a[Q], vw, df[NF];
+      lsa_terms<Q, CLS>(A, I, me, W.sorted[c0 + G.gl], kt, ih, a, vw, df);
+      double *row = W.u.rows[G.gl];
+#pragma unroll
+      for (int r = 0; r < Q; ++r) row[r] = a[r];
+      row[RL::VW] = vw;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) row[RL::DF + f] = df[f];
+    }
+    wave_sync();
+    S.add_chunk(W, nc, G.gl);
+  }
+  wave_sync();
+  double d[NF][5];
+  if (n >= Q) {  // LSSPH_A_mod.f90:208 (num_neighbors >= q)
+    row_dgesv<Q, NF>(W, S, G);
+    double X[NF][5];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) X[f][k] = W.u.lu.r[k][Q + f];
+    scale_derivs<NF>(X, I, d);
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) d[f][k] = 0.0;
+  }
+  if (G.gl == 0) finish<CLS, true>(A, I, A.items[s], s, me, d);
+}
+
+// Particle shifting with q = 20 on row groups.
+template <int Q, bool T>
+__global__ void __launch_bounds__(64) k_pst_rows(PstArgs A) {
+  constexpr int NF = T ? 4 : 3;  // u, v, rho (, tem)
+  using RL = RowLds<Q, NF, 2>;
+  constexpr int NG = RowGroup<Q>::G;
+  __shared__ RL lds_all[NG];
+  const RowGroup<Q> G(threadIdx.x);
+  if (G.g >= NG) return;
+  const int s = blockIdx.x * NG + G.g;
+  if (s >= A.n) return;
+  RL &W = lds_all[G.g];
+  const int32_t inf = A.sinfo[s];
+  const int ii = info_inst(inf);
+  const InstDev &I = A.inst[ii];
+  if (info_ptype(inf) != 0 || I.retired) {
+    if (G.gl == 0) A.mout[s] = A.mir[s];  // walls move on with the records
+    return;
+  }
+  const Rec me = A.mir[s];
+  const int n = group_neighbors(A.mir, A.cstart, A.sinfo, A.items, A.orig_of, A.mv.counters, I, s, me,
+                                W.u.scan.list, W.u.scan.key, W.sorted, G);
+  double ih[5];
+  inv_h_powers(I, ih);
+  const int kt = A.kt;
+  RowSums<Q, NF> S;
+  S.zero();
+  double sx = 0.0, sy = 0.0;
+  for (int c0 = 0; c0 < n; c0 += Q) {
+    const int nc = min(Q, n - c0);
+    wave_sync();
+    if (G.gl < nc) {
+      double a[Q], vw, df[NF], shx, shy;
+      pst_terms<Q, NF>(A, I, me, W.sorted[c0 + G.gl], kt, ih, a, vw, df, shx, shy);
+      double *row = W.u.rows[G.gl];
+#pragma unroll
+      for (int r = 0; r < Q; ++r) row[r] = a[r];
+      row[RL::VW] = vw;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) row[RL::DF + f] = df[f];
+      row[RL::XT] = shx;
+      row[RL::XT + 1] = shy;
+    }
+    wave_sync();
+    for (int k = 0; k < nc; ++k) {  // the shift sums in list order (:145-146)
+      sx = sx + W.u.rows[k][RL::XT];
+      sy = sy + W.u.rows[k][RL::XT + 1];
+    }
+    S.add_chunk(W, nc, G.gl);
+  }
+  wave_sync();
+  // shift vector and its clamp (:217-225)
+  const double umax = __longlong_as_double((long long)A.umax[ii]);
+  const double coe = I.pst_c * umax * I.dt * I.h;
+  double dxs = -coe * sx, dys = -coe * sy;
+  const double rs = sqrt(dxs * dxs + dys * dys);
+  if (rs > 0.2 * I.h) {
+    dxs = (0.2 * I.h) * dxs / rs;
+    dys = (0.2 * I.h) * dys / rs;
+  }
+  const bool solved = n >= Q;  // :240-247
+  if (solved) row_dgesv<Q, NF>(W, S, G);
+  if (G.gl != 0) return;
+  double a[Q];
+  basis_hq<Q>(dxs, dys, ih, a);
+  double acc[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    double t = 0.0;
+    if (solved) {
+#pragma unroll
+      for (int k = 0; k < Q; ++k) t += a[k] * W.u.lu.r[k][Q + f];
+    }
+    acc[f] = t;
+  }
+  Rec out = me;
+  const bool per = I.periodic;
+  out.x = wrap_pos(me.x + dxs, I.len_x, per);
+  out.y = wrap_pos(me.y + dys, I.len_y, per);
+  out.u = me.u + acc[0];
+  out.v = me.v + acc[1];
+  out.rho = me.rho + acc[2];
+  if constexpr (T) out.tem = me.tem + acc[3];
+  out.pre = I.c_sq * (out.rho - I.rho_ref);
+  out.vol = I.sp_mass / out.rho;
+  A.mout[s] = out;
+  record_move(A.mv, I, s, out.x, out.y);
+}
+
+// ---------------------------------------------------------------------------
 // host launchers: groups of 64 / ppw lanes (ppw particles per wavefront)
 // ---------------------------------------------------------------------------
 template <int Q, int GS>
@@ -587,9 +990,23 @@ static void lsa_wave_g(hipStream_t s, const LsaArgs &a, int cls) {
     hipLaunchKernelGGL((k_lsa_wave<Q, 2, GS>), g, b, 0, s, a);
 }
 
+template <int Q>
+static void lsa_rows(hipStream_t s, const LsaArgs &a, int cls) {
+  constexpr int per_block = RowGroup<Q>::G;
+  const dim3 g((a.n + per_block - 1) / per_block), b(64);
+  if (cls == 0)
+    hipLaunchKernelGGL((k_lsa_rows<Q, 0>), g, b, 0, s, a);
+  else if (cls == 1)
+    hipLaunchKernelGGL((k_lsa_rows<Q, 1>), g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_lsa_rows<Q, 2>), g, b, 0, s, a);
+}
+
 void launch_lsa_wave(hipStream_t s, Device &d, const LsaArgs &a, int, int cls) {
   if (a.n == 0) return;
-  if (wave_particles(d) == 2)
+  if (wave_particles(d) == 3)
+    lsa_rows<20>(s, a, cls);
+  else if (wave_particles(d) == 2)
     lsa_wave_g<20, 32>(s, a, cls);
   else
     lsa_wave_g<20, 64>(s, a, cls);
@@ -605,10 +1022,22 @@ static void pst_wave_g(hipStream_t s, const PstArgs &a, bool tem) {
     hipLaunchKernelGGL((k_pst_wave<Q, false, GS>), g, b, 0, s, a);
 }
 
+template <int Q>
+static void pst_rows(hipStream_t s, const PstArgs &a, bool tem) {
+  constexpr int per_block = RowGroup<Q>::G;
+  const dim3 g((a.n + per_block - 1) / per_block), b(64);
+  if (tem)
+    hipLaunchKernelGGL((k_pst_rows<Q, true>), g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_pst_rows<Q, false>), g, b, 0, s, a);
+}
+
 void launch_pst_wave(hipStream_t s, Device &d, const PstArgs &a, int) {
   if (a.n == 0) return;
   const bool tem = d.opt.has_tem != 0;
-  if (wave_particles(d) == 2)
+  if (wave_particles(d) == 3)
+    pst_rows<20>(s, a, tem);
+  else if (wave_particles(d) == 2)
     pst_wave_g<20, 32>(s, a, tem);
   else
     pst_wave_g<20, 64>(s, a, tem);

@@ -107,9 +107,10 @@ struct Device {
 // for SPH_MODEL 3 handles created with precision LSSPH_MIXED.
 inline bool mixed_path(const Device &d) { return d.opt.precision == LSSPH_MIXED && d.opt.sph_model == 3; }
 
-// Particles per wavefront of the lane-group kernels of SPH_MODEL 6 (1 or
-// 2); the default is the faster setting measured at 1024^2.
-inline int wave_particles(const Device &d) { return d.wave_ppw ? d.wave_ppw : 2; }
+// Particles per wavefront of the SPH_MODEL 6 kernels: 1 or 2 (lane groups
+// of 64 or 32), 3 (row groups of 20 lanes); the default is the fastest
+// setting measured at 1024^2.
+inline int wave_particles(const Device &d) { return d.wave_ppw ? d.wave_ppw : 3; }
 
 // cells.hip
 void launch_fill_i32(hipStream_t s, int32_t *p, int32_t v, int n);

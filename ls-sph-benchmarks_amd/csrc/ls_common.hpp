@@ -582,8 +582,8 @@ struct PstArgs {
   int staged;  // wave-staged candidate scan allowed
 };
 
-// The LS operators of SPH_MODEL 6 (q = 20), one wavefront per particle
-// (highorder.hip).  cls: 0 diffusion, 1 flow, 2 Boussinesq.
+// The LS operators of SPH_MODEL 6 (q = 20) on lane groups or row groups
+// of a wavefront (highorder.hip).  cls: 0 diffusion, 1 flow, 2 Boussinesq.
 void launch_lsa_wave(hipStream_t s, Device &d, const LsaArgs &a, int sph_model, int cls);
 void launch_pst_wave(hipStream_t s, Device &d, const PstArgs &a, int sph_model);
 

@@ -673,31 +673,39 @@ def test_mixed_request_runs_fp64_for_other_models(model):
             assert np.array_equal(da[k], db[k]), k
 
 
-@pytest.mark.parametrize("model,ppw", [(6, 1), (6, 2)])
-@pytest.mark.parametrize("box", ["periodic", "walls"])
+@pytest.mark.parametrize("model,ppw", [(6, 1), (6, 2), (6, 3)])
+@pytest.mark.parametrize("box", ["periodic", "walls", "bous", "heat"])
 def test_wave_particles_do_not_change_results(model, ppw, box):
-    """SPH_MODEL 6 with 1 or 2 particles per wavefront (lane groups of 64
-    or 32): the same results bit for bit as the default setting, over two
-    RK4 + PST steps (walls: the ghost fill in between)."""
+    """SPH_MODEL 6 with 1, 2 or 3 particles per wavefront (lane groups of 64
+    or 32 lanes, or row groups of 20): the same results bit for bit as the
+    default setting, over two RK4 + PST steps (walls: the ghost fill in
+    between), for the flow, convection (temperature column) and heat
+    (one field, no shifting) operators."""
     from lssph import config as lcfg
     if box == "periodic":
         base, st = workloads.taylor_green_periodic(n=28, re=100.0, rk=4, jitter=0.2, seed=70 + model)
-    else:
+    elif box == "walls":
         base, st = workloads.taylor_green_box(n=24, re=100.0, rk=4, jitter=0.2, seed=70 + model)
+    elif box == "bous":
+        base, st = workloads.boussinesq(n=24, ra=1e5, jitter=0.2, seed=70 + model)
+    else:
+        base, st = workloads.heat_periodic(n=28, kappa=0.01, jitter=0.1, seed=70 + model)
     param = lcfg.init_param(base.cfg.replace(sph_model=model))
     print(f"[gpu] SPH_MODEL {model}, {ppw} particles per wavefront, {box}", flush=True)
     out = []
     for setting in (0, ppw):
         with simulation_from_state(param, st, precision=0) as s:
             s.set_wave_particles(setting)
-            if box == "walls":
+            if box in ("walls", "bous"):
                 s.ghost()
             rhs = s.calculate_RHS()
             s.step(2)
             out.append((rhs, s.download()))
-    for k in ("u", "v", "rho"):
+    keys = ("u",) if box == "heat" else ("u", "v", "rho")
+    for k in keys:
         assert np.array_equal(out[0][0][k], out[1][0][k]), k
-    for k in ("x", "y", "u", "v", "rho", "pre"):
+    fields = ("x", "y", "u", "v", "rho", "pre") + (("tem",) if box == "bous" else ())
+    for k in fields:
         assert np.array_equal(out[0][1][k], out[1][1][k]), k
 
 
@@ -707,7 +715,7 @@ def test_wave_particles_rejects_bad_values():
     base, st = workloads.taylor_green_periodic(n=16, re=100.0, rk=4, jitter=0.0, seed=0)
     param = lcfg.init_param(base.cfg.replace(sph_model=6))
     with simulation_from_state(param, st, precision=0) as s:
-        for bad in (3, 4, -1):  # 1 or 2 (0: the default)
+        for bad in (4, -1):  # 1, 2 or 3 (0: the default)
             with pytest.raises(LssphError):
                 s.set_wave_particles(bad)
 

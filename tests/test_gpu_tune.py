@@ -68,12 +68,13 @@ def test_autotune_particles_per_wavefront_of_sph_model_6():
     def make(skin):
         return simulation_from_state(param, st, precision=0, skin=skin)
 
-    print("[gpu] autotune, SPH_MODEL 6, particles per wavefront 1 and 2", flush=True)
-    r = tune.autotune(make, steps=1, blocks=(128,), skins=(0.08,), wave_particles=(1, 2),
+    print("[gpu] autotune, SPH_MODEL 6, particles per wavefront 1, 2 and 3", flush=True)
+    r = tune.autotune(make, steps=1, blocks=(128,), skins=(0.08,), wave_particles=(1, 2, 3),
                       default=(128, 0, 0.08, 1))
     print(f"[gpu] tuned: {r.as_dict()}", flush=True)
-    assert {s.wave_particles for s in r.table if not s.error} == {1, 2}
+    assert {s.wave_particles for s in r.table if not s.error} == {1, 2, 3}
     a = _run(make, tune.Setting(128, 0, 0.08, 1), 2)
-    b = _run(make, tune.Setting(128, 0, 0.08, 2), 2)
-    for k in ("x", "y", "u", "v", "rho", "pre"):
-        assert np.array_equal(a[k], b[k]), k
+    for ppw in (2, 3):
+        b = _run(make, tune.Setting(128, 0, 0.08, ppw), 2)
+        for k in ("x", "y", "u", "v", "rho", "pre"):
+            assert np.array_equal(a[k], b[k]), (ppw, k)

@@ -22,8 +22,8 @@ class FakeSim:
         self.clock = 0.0
 
     def set_wave_particles(self, ppw):
-        if ppw not in (0, 1, 2):
-            raise LssphError(-1, "lssph_set_wave_particles: 0, 1 or 2")
+        if ppw not in (0, 1, 2, 3):
+            raise LssphError(-1, "lssph_set_wave_particles: 0, 1, 2 or 3")
         self.ppw = ppw
 
     def set_tuning(self, block=0, list_len=0, reorder_interval=0):
