@@ -681,15 +681,25 @@ struct RowSums {
 #pragma unroll
     for (int f = 0; f < NF; ++f) bacc[f] = 0.0;
   }
+  // Neighbor k's operands are read into registers together before they are
+  // used (left to itself the scheduler reuses two registers for the reads
+  // and waits on each).
   template <class RL>
   __device__ __forceinline__ void add_chunk(const RL &W, int nc, int i) {
     for (int k = 0; k < nc; ++k) {
       const double *row = W.u.rows[k];
-      const double vwa = row[RL::VW] * row[i];  // V_j W a_i
+      double a[Q], g[1 + NF];
 #pragma unroll
-      for (int r = 0; r < Q; ++r) acc[r] += vwa * row[r];
+      for (int r = 0; r < Q; ++r) a[r] = row[r];
 #pragma unroll
-      for (int f = 0; f < NF; ++f) bacc[f] += vwa * row[RL::DF + f];
+      for (int f = 0; f <= NF; ++f) g[f] = row[RL::VW + f];
+      const double ai = row[i];
+      __builtin_amdgcn_sched_barrier(0);
+      const double vwa = g[0] * ai;  // V_j W a_i
+#pragma unroll
+      for (int r = 0; r < Q; ++r) acc[r] += vwa * a[r];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) bacc[f] += vwa * g[1 + f];
     }
   }
 };
