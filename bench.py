@@ -418,7 +418,7 @@ def main():
     ap.add_argument("--sph-model", type=int, default=3,
                     help="1-2 classical SPH, 4-6 LS q=9/14/20: time that operator (FP64) instead of the headline")
     ap.add_argument("--wave-particles", type=int, default=0,
-                    help="particles per wavefront of the SPH_MODEL 5-6 kernels (0: the library's default)")
+                    help="particles per wavefront of the SPH_MODEL 6 kernels: 1, 2 or 3 (0: the library's default, 3)")
     ap.add_argument("--cell-split", type=int, default=0,
                     help="cell columns per stencil reach in x (0: the library's default)")
     ap.add_argument("--workload", choices=("taylor-green", "ensemble", "boussinesq"), default="taylor-green",
