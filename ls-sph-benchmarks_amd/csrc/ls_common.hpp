@@ -205,9 +205,9 @@ __device__ __forceinline__ double sym5(const R (&M)[15], int i, int c) {
 // system, kappa_1 = ||M||_1 ||M^-1||_1, exceeds cond_max (or is not finite).
 // kappa_1 is exact here, not an estimate that can undershoot: M^-1 =
 // L^-T D^-1 L^-1 is formed from the factors (about 130 FP64 operations per
-// particle; ||M||_1 is taken on the FP32 sums) -- unless a Gershgorin bound
-// already puts kappa_1 below half of cond_max (about 40 operations; the
-// usual case for interior particles).  The basis5 system is
+// particle; ||M||_1 is taken on the FP32 sums) -- unless a diagonal-dominance
+// bound already puts kappa_1 below half of cond_max (about 40 operations;
+// the usual case for interior particles).  The basis5 system is
 // D M D with D = diag(1, 1, 1/2, 1/2, 1),
 // so its entries and those of its inverse are the basis5s ones times
 // D_i D_j and 1 / (D_i D_j).  (The pivot ratio max(d) / min(d) that this
@@ -260,13 +260,14 @@ __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, do
     }
     nrm_m = fmaxf(nrm_m, col * float(basis5_weight(j)));
   }
-  // A sufficient test first.  By Gershgorin, every eigenvalue of the
-  // symmetric basis5 system B is at least g = min_i (B_ii - sum_j!=i |B_ij|);
-  // when g > 0, ||B^-1||_1 <= sqrt(5) ||B^-1||_2 <= sqrt(5) / g.  A system
-  // whose bound ||B||_1 sqrt(5) / g is at most half of cond_max has
-  // kappa_1 < cond_max (the factor 2 covers the rounding of both sides), so
-  // it passes the exact test below, which then need not run: the decisions,
-  // and so the results, are those of the exact test alone.
+  // A sufficient test first.  When the symmetric basis5 system B is
+  // strictly diagonally dominant, g = min_i (B_ii - sum_j!=i |B_ij|) > 0,
+  // Varah's bound (Linear Algebra Appl. 11, 1975) gives ||B^-1||_inf <= 1 / g,
+  // and ||B^-1||_1 = ||B^-1||_inf for symmetric B, so kappa_1 <= ||B||_1 / g.
+  // A system whose bound is at most half of cond_max has kappa_1 < cond_max
+  // (the factor 2 covers the rounding of both sides), so it passes the
+  // exact test below, which then need not run: the decisions, and so the
+  // results, are those of the exact test alone.
   double g = 1.0e300, bmax = 0.0;
 #pragma unroll
   for (int i = 0; i < 5; ++i) {
@@ -278,7 +279,7 @@ __device__ __forceinline__ bool ldlt_solve_refine(const LsSums<float, NF> &S, do
     g = fmin(g, bii - off);
     bmax = fmax(bmax, bii);
   }
-  const bool bounded = g >= 1.0e-3 * bmax && double(nrm_m) * 2.2360679775 <= 0.5 * cond_max * g;
+  const bool bounded = g >= 1.0e-3 * bmax && double(nrm_m) <= 0.5 * cond_max * g;
   if (!bounded) {
     // ||M^-1||_1 one column at a time: column j solves M y = e_j with the
     // factors (L y' = e_j, y' /= d, L^T y = y'), so only five doubles of the
