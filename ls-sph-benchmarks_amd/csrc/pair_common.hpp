@@ -57,6 +57,33 @@ __device__ __forceinline__ NbList no_list() {
   return L;
 }
 
+// Minimum and maximum of a value over a full wavefront (every lane active),
+// by DPP: row shifts by 1, 2, 4 and 8 leave each 16-lane row's extreme in its
+// lane 15, and the row broadcasts 15 and 31 carry those on to lane 63, whose
+// value is returned (wave-uniform).  Lanes without a source in a step keep
+// the identity.
+__device__ __forceinline__ int wave_min_full(int v) {
+  constexpr int kId = 0x7fffffff;
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ int wave_max_full(int v) {
+  constexpr int kId = -0x7fffffff - 1;
+  v = max(v, __builtin_amdgcn_update_dpp(kId, v, 0x111, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(kId, v, 0x112, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(kId, v, 0x114, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(kId, v, 0x118, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(kId, v, 0x142, 0xa, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(kId, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Orders the LDS accesses of the lanes of one wavefront (LDS operations of
 // a wavefront complete in issue order; the fences keep the compiler from
 // moving accesses across this point).
@@ -254,20 +281,12 @@ __device__ __forceinline__ bool scan_staged(const Rec *__restrict__ mir, const i
   const int off = __builtin_amdgcn_readfirstlane(I.cell_off);
   const int cx = lc % I.nx, cy = lc / I.nx;
   if (__ballot(I.cell_off != off || stencil_wraps(I, lc)) != 0ull) return false;
-  int mn = cx, mx = cx, ymn = cy, ymx = cy;
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    mn = min(mn, __shfl_xor(mn, m));
-    mx = max(mx, __shfl_xor(mx, m));
-    ymn = min(ymn, __shfl_xor(ymn, m));
-    ymx = max(ymx, __shfl_xor(ymx, m));
-  }
   // particles that crossed into the next cell row since the last build put
   // a wave on two rows: four candidate rows, three of them per lane
-  const int cy0 = __builtin_amdgcn_readfirstlane(ymn), cy1 = __builtin_amdgcn_readfirstlane(ymx);
+  const int cy0 = wave_min_full(cy), cy1 = wave_max_full(cy);
   if (cy1 - cy0 > 1) return false;
-  const int x0 = max(__builtin_amdgcn_readfirstlane(mn) - kx, 0);
-  const int x1 = min(__builtin_amdgcn_readfirstlane(mx) + kx, nx - 1);
+  const int x0 = max(wave_min_full(cx) - kx, 0);
+  const int x1 = min(wave_max_full(cx) + kx, nx - 1);
   const int y0 = max(cy0 - 1, 0), y1 = min(cy1 + 1, ny - 1);
   const int cap = 8 * (L.list_len + 1);  // 16-byte positions in the upper half of the wave's rows
   const int base = cstart[off + y0 * nx + x0];
