@@ -674,13 +674,15 @@ def test_mixed_request_runs_fp64_for_other_models(model):
 
 
 @pytest.mark.parametrize("model,ppw", [(6, 1), (6, 2), (6, 3)])
-@pytest.mark.parametrize("box", ["periodic", "walls", "bous", "heat"])
+@pytest.mark.parametrize("box", ["periodic", "walls", "bous", "heat", "quintic"])
 def test_wave_particles_do_not_change_results(model, ppw, box):
     """SPH_MODEL 6 with 1, 2 or 3 particles per wavefront (lane groups of 64
     or 32 lanes, or row groups of 20): the same results bit for bit as the
     default setting, over two RK4 + PST steps (walls: the ghost fill in
     between), for the flow, convection (temperature column) and heat
-    (one field, no shifting) operators."""
+    (one field, no shifting) operators, and with the quintic spline, whose
+    support (h_eff = 4.65 dx, about 68 neighbors) takes the sums over four
+    chunks of neighbors."""
     from lssph import config as lcfg
     if box == "periodic":
         base, st = workloads.taylor_green_periodic(n=28, re=100.0, rk=4, jitter=0.2, seed=70 + model)
@@ -688,6 +690,9 @@ def test_wave_particles_do_not_change_results(model, ppw, box):
         base, st = workloads.taylor_green_box(n=24, re=100.0, rk=4, jitter=0.2, seed=70 + model)
     elif box == "bous":
         base, st = workloads.boussinesq(n=24, ra=1e5, jitter=0.2, seed=70 + model)
+    elif box == "quintic":
+        base, st = workloads.taylor_green_periodic(n=32, re=100.0, rk=4, jitter=0.2, seed=80 + model,
+                                                   kernel_type=2)
     else:
         base, st = workloads.heat_periodic(n=28, kappa=0.01, jitter=0.1, seed=70 + model)
     param = lcfg.init_param(base.cfg.replace(sph_model=model))
