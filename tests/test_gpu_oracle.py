@@ -449,6 +449,38 @@ def test_mixed_ensemble_equals_separate_handles_bitwise(kind):
             assert np.array_equal(got[k], ref[k]), k
 
 
+@pytest.mark.parametrize("ppw", [2, 3])
+def test_q20_ensemble_equals_separate_handles_bitwise(ppw):
+    """SPH_MODEL 6 with lane groups (two particles per wavefront) or row
+    groups (three): the groups of one wavefront may hold particles of
+    different instances, each with its own constants and cells.  Four
+    cavities (h_eff = 3.1 dx, about 30 neighbors) stepped together equal each
+    stepped in a handle of its own, bit for bit."""
+    from lssph import config as lcfg
+    params, states, meta = workloads.cavity_ensemble(count=4, sizes=(12, 16, 20), seed=7, precision=0,
+                                                     coe_h=1.55)
+    params = [lcfg.init_param(p.cfg.replace(sph_model=6)) for p in params]
+    print(f"[gpu] q = 20 ensemble, {ppw} particles per wavefront, n_int={[st.num_int for st in states]}",
+          flush=True)
+    ni = [s.num_int for s in states]
+    ne = [s.num_ext for s in states]
+    with Simulation(params, ni, ne, precision=0) as sim:
+        sim.set_wave_particles(ppw)
+        sim.upload(states)
+        sim.init_cell()
+        sim.ghost()
+        sim.step(2)
+        out = split_states(sim.download(), ni, ne)
+    for p, st, got in zip(params, states, out):
+        with simulation_from_state(p, st, precision=0) as one:
+            one.set_wave_particles(ppw)
+            one.ghost()
+            one.step(2)
+            ref = one.download()
+        for k in ("x", "y", "u", "v", "rho", "pre"):
+            assert np.array_equal(got[k], ref[k]), k
+
+
 def test_heat_ensemble_steady_check_per_instance():
     """Diffusion boxes of different sizes and diffusivities in one handle:
     fields and the per-instance steady-state maxima max|f - f_prev| equal
