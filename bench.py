@@ -229,7 +229,7 @@ MODEL_NAMES = {1: "classical SPH (pressure sum model)", 2: "classical SPH (diffe
                4: "LS q=9 (3rd-order Taylor)", 5: "LS q=14 (4th-order Taylor)", 6: "LS q=20 (5th-order Taylor)"}
 
 
-def model_bench(args) -> None:
+def model_bench(args, emit: bool = True) -> dict:
     """The same periodic Taylor-Green workload (RK4 + PST every step) with one
     of the other operators, FP64; a line of its own, not the headline."""
     from lssph import config as lcfg
@@ -241,7 +241,7 @@ def model_bench(args) -> None:
                 setup=(lambda sim: sim.set_wave_particles(ppw)) if ppw else None)
     kt = r["kt"]
     n_part = st.num_int
-    print(json.dumps({
+    line = {
         "metric": f"particle-steps/s, SPH_MODEL {args.sph_model}", "value": round(n_part * args.steps / (r["ms"] / 1e3), 1),
         "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(r["ms"] / args.steps, 4), "higher_is_better": True, "dtype": "f64", "data": "synthetic",
@@ -253,7 +253,10 @@ def model_bench(args) -> None:
         "operator_ms_per_launch": {"rhs": round(kt.lsa_ms / max(kt.lsa_launches, 1), 4),
                                    "pst": round(kt.pst_ms / max(kt.pst_launches, 1), 4)},
         "wave_particles": ppw or "default",
-        "tg_l2_error": r["l2"]}), flush=True)
+        "tg_l2_error": r["l2"]}
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def _timed_run(make, steps: int, warmup: int, graph: bool, post=None) -> dict:
@@ -437,6 +440,19 @@ def other_configurations(args) -> dict:
             for key, src in (("mixed_vs_fp64", "mixed_vs_fp64"), ("fp64_perturbed_vs_fp64", "fp64_perturbed_vs_fp64")):
                 summary[key] = {k: line[src][k] for k in ("u", "v", "tem")}
         out[name] = summary
+    # the other operators on the 1024^2 Taylor-Green, FP64, 3 steps each (the
+    # q = 20 scheme is not stable much longer at this support)
+    ops = {}
+    for model in (2, 4, 5, 6):
+        sub = argparse.Namespace(**vars(args))
+        sub.n, sub.steps, sub.warmup, sub.sph_model, sub.wave_particles = 1024, 3, 1, model, 0
+        try:
+            line = model_bench(sub, emit=False)
+            ops[f"sph_model_{model}"] = {"operator": MODEL_NAMES[model], "ms_per_step": line["ms_per_step"],
+                                         "operator_ms_per_launch": line["operator_ms_per_launch"]}
+        except Exception as e:  # noqa: BLE001 -- reported in the line
+            ops[f"sph_model_{model}"] = {"error": f"{type(e).__name__}: {e}"}
+    out["other_operators_tg1024_fp64"] = ops
     return out
 
 
